@@ -1,0 +1,317 @@
+// k_field.hip -- FP64 spectral Poisson solve and electric field, hand-written LDS FFT.
+//
+// Reference stages replaced (thesisCode/collisionless/double/src/pic.cu):
+//   cufftExecD2Z :551, fftPoissonSolver :147-194, cufftExecZ2D :557,
+//   normaliseTransform :218-232, copyD2D :203-210, computeElectricField :242-283.
+// Three kernels per step:
+//   rows    : fixed-point rho (two rows packed as one complex row) -> row spectra,
+//             written transposed as S[ky][x]; clears the other rho parity buffer
+//   columns : column FFT, phi_hat = rho_hat / |k|^2 (k = 2 pi m / L, mean removed),
+//             Ex_hat = -i sin(kx dx)/dx phi_hat, Ey_hat likewise (the Fourier symbol of the
+//             periodic central difference), inverse column FFTs
+//   field   : Ex + i Ey from one inverse complex row FFT per row; writes the padded
+//             (Ex, Ey) node array and the field energy (fixed-order reduction)
+#include <type_traits>
+#include "launch.h"
+
+namespace {
+
+__device__ __forceinline__ int bitrev(int v, int logn) {
+  return (int)(__brev((unsigned)v) >> (32 - logn));
+}
+
+// radix-2 DIT on LDS data stored in bit-reversed order; sign -1 forward, +1 inverse
+__device__ void fft_lds(double2* a, int n, int logn, int sign, const double2* __restrict__ tw) {
+  const double ws = (sign < 0) ? 1.0 : -1.0;
+  for (int s = 1; s <= logn; ++s) {
+    const int h = 1 << (s - 1);
+    const int ts = PIC_FFT_MAX >> s;
+    for (int b = threadIdx.x; b < (n >> 1); b += PIC_WG) {
+      const int j = b & (h - 1);
+      const int i0 = ((b >> (s - 1)) << s) + j;
+      const int i1 = i0 + h;
+      double2 w = tw[j * ts];
+      w.y *= ws;
+      const double2 u = a[i0], v = a[i1];
+      const double tr = fma(v.x, w.x, -v.y * w.y);
+      const double ti = fma(v.x, w.y, v.y * w.x);
+      a[i0] = make_double2(u.x + tr, u.y + ti);
+      a[i1] = make_double2(u.x - tr, u.y - ti);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ double fixed_to_density(unsigned long long hi, unsigned long long lo,
+                                                   int fp64, double scale) {
+  if (fp64)
+    return ((double)(long long)hi * 9.313225746154785e-10 +
+            (double)(long long)lo * 8.673617379884035e-19) * scale;
+  return (double)(long long)hi * scale;
+}
+
+// sin(2 pi m / n) from the twiddle table (tw[j] = exp(-2 pi i j / PIC_FFT_MAX))
+__device__ __forceinline__ double sin2pi_frac(int m, int n, const double2* __restrict__ tw) {
+  const int st = PIC_FFT_MAX / n;
+  if (m == 0 || 2 * m == n) return 0.0;
+  if (2 * m < n) return -tw[m * st].y;
+  return tw[(n - m) * st].y;
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
+  extern __shared__ double2 lds[];
+  const int2 it = a.row_pair_items[blockIdx.x];
+  const RunDesc rd = a.runs[it.x];
+  const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
+  const int i0 = it.y;
+  const long long g0 = rd.gbase + (long long)i0 * ny, g1 = g0 + ny;
+  if (a.rho_in) {
+    const double* r = a.rho_in + (g0 - rd.gbase);
+    for (int j = threadIdx.x; j < ny; j += PIC_WG)
+      lds[bitrev(j, logny)] = make_double2(r[j], r[j + ny]);
+  } else {
+    const int par = (int)(a.st[it.x].step & 1);
+    const unsigned long long* rh = a.rho + par * a.gsum;
+    unsigned long long* zw = a.rho_w + (1 - par) * a.gsum;
+    const unsigned long long* rl = a.fp64 ? a.rho_lo + par * a.gsum : nullptr;
+    unsigned long long* zl = a.fp64 ? a.rho_lo_w + (1 - par) * a.gsum : nullptr;
+    for (int j = threadIdx.x; j < ny; j += PIC_WG) {
+      const double d0 = fixed_to_density(rh[g0 + j], a.fp64 ? rl[g0 + j] : 0ull, a.fp64, rd.rho_scale);
+      const double d1 = fixed_to_density(rh[g1 + j], a.fp64 ? rl[g1 + j] : 0ull, a.fp64, rd.rho_scale);
+      lds[bitrev(j, logny)] = make_double2(d0, d1);
+      zw[g0 + j] = 0ull;
+      zw[g1 + j] = 0ull;
+      if (a.fp64) { zl[g0 + j] = 0ull; zl[g1 + j] = 0ull; }
+    }
+  }
+  __syncthreads();
+  fft_lds(lds, ny, logny, -1, a.tw);
+  double2* S = a.S + rd.sbase;
+  for (int k = threadIdx.x; k <= ny / 2; k += PIC_WG) {
+    const double2 zk = lds[k], zm = lds[(ny - k) & (ny - 1)];
+    const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+    const double2 B = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    S[(long long)k * nx + i0] = A;
+    S[(long long)k * nx + i0 + 1] = B;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
+  extern __shared__ double2 lds[];
+  const int2 it = a.col_items[blockIdx.x];
+  const RunDesc rd = a.runs[it.x];
+  const int nx = rd.nx, ny = rd.ny, lognx = rd.lognx;
+  const int k = it.y;
+  double2* col = lds;
+  double2* A = lds + nx;
+  double2* B = lds + 2 * nx;
+  double2* C = lds + 3 * nx;
+  double2* S = a.S + rd.sbase + (long long)k * nx;
+  for (int i = threadIdx.x; i < nx; i += PIC_WG) col[bitrev(i, lognx)] = S[i];
+  __syncthreads();
+  fft_lds(col, nx, lognx, -1, a.tw);
+  if (k == 0 && threadIdx.x == 0 && a.diag && !a.rho_in) {
+    const double2 r1 = col[1];
+    const long long stp = a.st[it.x].step;
+    a.diag[(rd.dbase + (stp % PIC_DIAG_CAP)) * 8 + 4] =
+        sqrt(r1.x * r1.x + r1.y * r1.y) / ((double)nx * ny);
+  }
+  const double twopi = 6.283185307179586476925;
+  const int kk = (2 * k < ny) ? k : k - ny;
+  const double ky = twopi * kk / rd.ly;
+  const double sy = sin2pi_frac(k, ny, a.tw) / rd.dy;
+  for (int m = threadIdx.x; m < nx; m += PIC_WG) {
+    const int mm = (2 * m < nx) ? m : m - nx;
+    const double kx = twopi * mm / rd.lx;
+    const double k2 = kx * kx + ky * ky;
+    const double2 r = col[m];
+    double2 ph = make_double2(0.0, 0.0);
+    if (m != 0 || k != 0) ph = make_double2(r.x / k2, r.y / k2);
+    const double sx = sin2pi_frac(m, nx, a.tw) / rd.dx;
+    const int br = bitrev(m, lognx);
+    A[br] = make_double2(sx * ph.y, -sx * ph.x);
+    B[br] = make_double2(sy * ph.y, -sy * ph.x);
+    if (a.want_phi) C[br] = ph;
+  }
+  __syncthreads();
+  fft_lds(A, nx, lognx, +1, a.tw);
+  fft_lds(B, nx, lognx, +1, a.tw);
+  if (a.want_phi) fft_lds(C, nx, lognx, +1, a.tw);
+  double2* Sx = a.Sx + rd.sbase + (long long)k * nx;
+  double2* Sy = a.Sy + rd.sbase + (long long)k * nx;
+  for (int i = threadIdx.x; i < nx; i += PIC_WG) {
+    Sx[i] = A[i];
+    Sy[i] = B[i];
+    if (a.want_phi) S[i] = C[i];
+  }
+}
+
+// MODE 0: row i of (Ex + i Ey) -> padded E + field energy.  MODE 1: rows (i, i+1) of phi.
+template <typename R, int MODE>
+__global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[4];
+  __shared__ unsigned int ticket_s;
+  const int2 it = (MODE == 0) ? a.row_items[blockIdx.x] : a.row_pair_items[blockIdx.x];
+  const RunDesc rd = a.runs[it.x];
+  const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
+  const int i = it.y;
+  const double2* SA = (MODE == 0) ? a.Sx + rd.sbase + i : a.S + rd.sbase + i;
+  const double2* SB = (MODE == 0) ? a.Sy + rd.sbase + i : a.S + rd.sbase + i + 1;
+  for (int k = threadIdx.x; k < ny; k += PIC_WG) {
+    double2 A, B;
+    if (2 * k <= ny) {
+      A = SA[(long long)k * nx];
+      B = SB[(long long)k * nx];
+      if (k == 0 || 2 * k == ny) { A.y = 0.0; B.y = 0.0; }
+    } else {
+      const int kk = ny - k;
+      A = SA[(long long)kk * nx];
+      B = SB[(long long)kk * nx];
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    lds[bitrev(k, logny)] = make_double2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  fft_lds(lds, ny, logny, +1, a.tw);
+  const double norm = 1.0 / ((double)nx * ny);
+  if (MODE == 1) {
+    double* ph = a.phi + rd.gbase;
+    for (int j = threadIdx.x; j < ny; j += PIC_WG) {
+      ph[(long long)i * ny + j] = lds[j].x * norm;
+      ph[(long long)(i + 1) * ny + j] = lds[j].y * norm;
+    }
+    return;
+  }
+  typedef typename std::conditional<sizeof(R) == 4, float2, double2>::type R2;
+  R2* E = reinterpret_cast<R2*>(a.E) + rd.ebase;
+  double en = 0.0;
+  for (int j = threadIdx.x; j < ny; j += PIC_WG) {
+    const double ex = lds[j].x * norm, ey = lds[j].y * norm;
+    en += ex * ex + ey * ey;
+    R2 v;
+    v.x = (R)ex;
+    v.y = (R)ey;
+    E[(long long)i * (ny + 1) + j] = v;
+    if (j == 0) E[(long long)i * (ny + 1) + ny] = v;
+    if (i == 0) {
+      E[(long long)nx * (ny + 1) + j] = v;
+      if (j == 0) E[(long long)nx * (ny + 1) + ny] = v;
+    }
+  }
+  if (!a.diag) return;
+  const double s = block_sum(en, red);
+  if (threadIdx.x == 0) {
+    a.partials[blockIdx.x] = s;
+    __threadfence();
+    ticket_s = atomicAdd(a.ticket, 1u);
+  }
+  __syncthreads();
+  if (ticket_s != gridDim.x - 1) return;
+  __threadfence();
+  for (int r = 0; r < a.nruns; ++r) {
+    const RunDesc d = a.runs[r];
+    double v = 0.0;
+    for (int q = d.row_item_begin + threadIdx.x; q < d.row_item_end; q += PIC_WG) v += a.partials[q];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) {
+      const long long stp = a.st[r].step;
+      a.diag[(d.dbase + (stp % PIC_DIAG_CAP)) * 8 + 0] = 0.5 * v * d.dx * d.dy;
+    }
+  }
+  if (threadIdx.x == 0) *a.ticket = 0u;
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_op_efield(const double* __restrict__ phi,
+                                                      double* __restrict__ ex,
+                                                      double* __restrict__ ey, int nx, int ny,
+                                                      double dx, double dy) {
+  const long long G = (long long)nx * ny;
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
+       q += (long long)gridDim.x * PIC_WG) {
+    const int i = (int)(q / ny), j = (int)(q % ny);
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    ex[q] = (phi[(long long)im * ny + j] - phi[(long long)ip * ny + j]) / (2.0 * dx);
+    ey[q] = (phi[(long long)i * ny + jm] - phi[(long long)i * ny + jp]) / (2.0 * dy);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_unpad_E(const R* __restrict__ E, int nx, int ny,
+                                                    double* __restrict__ ex,
+                                                    double* __restrict__ ey) {
+  const long long G = (long long)nx * ny;
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
+       q += (long long)gridDim.x * PIC_WG) {
+    const long long i = q / ny, j = q % ny;
+    const long long e = i * (ny + 1) + j;
+    ex[q] = (double)E[2 * e];
+    ey[q] = (double)E[2 * e + 1];
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_rho_to_double(const RunDesc* __restrict__ runs,
+                                                          const RunState* __restrict__ st,
+                                                          int run,
+                                                          const unsigned long long* rho,
+                                                          const unsigned long long* rlo,
+                                                          long long gsum, int fp64,
+                                                          double background,
+                                                          double* __restrict__ out) {
+  const RunDesc rd = runs[run];
+  const int par = (int)(st[run].step & 1);
+  const long long G = (long long)rd.nx * rd.ny;
+  const unsigned long long* h = rho + par * gsum + rd.gbase;
+  const unsigned long long* l = fp64 ? rlo + par * gsum + rd.gbase : nullptr;
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
+       q += (long long)gridDim.x * PIC_WG)
+    out[q] = fixed_to_density(h[q], fp64 ? l[q] : 0ull, fp64, rd.rho_scale) + background;
+}
+
+inline int grid_for(long long n) {
+  long long b = (n + PIC_WG - 1) / PIC_WG;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+}  // namespace
+
+void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev) {
+  const size_t lrow = (size_t)a.max_ny * sizeof(double2);
+  const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  k_fft_rows<<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  k_fft_cols<<<a.n_col_items, PIC_WG, lcol, s>>>(a);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  if (a.fp64)
+    k_fft_field<double, 0><<<a.n_row_items, PIC_WG, lrow, s>>>(a);
+  else
+    k_fft_field<float, 0><<<a.n_row_items, PIC_WG, lrow, s>>>(a);
+  if (a.want_phi && a.phi) k_fft_field<float, 1><<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);
+  if (ev) (void)hipEventRecord(ev[3], s);
+}
+
+void launch_op_efield(hipStream_t s, const double* phi, double* ex, double* ey, int nx, int ny,
+                      double lx, double ly) {
+  k_op_efield<<<grid_for((long long)nx * ny), PIC_WG, 0, s>>>(phi, ex, ey, nx, ny, lx / nx,
+                                                             ly / ny);
+}
+
+void launch_unpad_E(hipStream_t s, const void* E, int fp64, int nx, int ny, double* ex,
+                    double* ey) {
+  if (fp64)
+    k_unpad_E<double><<<grid_for((long long)nx * ny), PIC_WG, 0, s>>>((const double*)E, nx, ny,
+                                                                      ex, ey);
+  else
+    k_unpad_E<float><<<grid_for((long long)nx * ny), PIC_WG, 0, s>>>((const float*)E, nx, ny, ex,
+                                                                     ey);
+}
+
+void launch_rho_to_double(hipStream_t s, const RunDesc* runs, RunState* st, int run,
+                          const unsigned long long* rho, const unsigned long long* rho_lo,
+                          long long gsum, int fp64, double background, double* out) {
+  k_rho_to_double<<<512, PIC_WG, 0, s>>>(runs, st, run, rho, rho_lo, gsum, fp64, background,
+                                         out);
+}

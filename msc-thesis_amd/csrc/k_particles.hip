@@ -1,0 +1,865 @@
+// k_particles.hip -- particle-side kernels: cell sort, fused gather+push+deposit, outputs.
+//
+// Reference stages replaced (thesisCode/collisionless/double/src/pic.cu):
+//   currentCell        :69-87    -> cell key computed inside the sort and the push
+//   chargeAssignment   :99-138   -> deterministic fixed-point CIC deposit in LDS windows
+//   forceCalculation   :298-355  -> bilinear gather fused into the push
+//   positionUpdate     :372-420  -> leapfrog push, periodic wrap into [0, L)
+//   cellTrackerSetup   thesisCode/dsmcCollisions/cellTrackerAdvanced/src/pic.cu:419-445
+//                                -> stable counting sort by key cx*ny+cy (ties by id)
+#include <climits>
+#include <type_traits>
+#include "launch.h"
+
+namespace {
+
+// ========================================================================== sort
+#define HWIN 8192  // LDS histogram window (keys) of the rank kernel
+
+template <typename R, bool FROM_POS>
+__global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict__ runs,
+                                                      const RunState* __restrict__ st,
+                                                      const R* __restrict__ x,
+                                                      const R* __restrict__ y,
+                                                      const int* __restrict__ keys_in,
+                                                      int* __restrict__ keys,
+                                                      int* __restrict__ rank,
+                                                      int* __restrict__ counts,
+                                                      const int2* __restrict__ items) {
+  __shared__ int hist[HWIN];
+  __shared__ int red[8];
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int np = st[it.x].np;
+  const int p0 = it.y, p1 = min(p0 + PIC_CHUNK, np);
+  if (p0 >= p1) return;  // uniform across the workgroup
+  const long long pb = rd.pbase;
+  const int nx = rd.nx, ny = rd.ny;
+  const R dxr = (R)rd.dx, dyr = (R)rd.dy;
+  int* cnt = counts + rd.cbase;
+  int kmin = INT_MAX, kmax = INT_MIN;
+  for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) {
+    int key;
+    if (FROM_POS) {
+      int cx, cy;
+      R f;
+      locate(x[pb + p], dxr, nx, cx, f);
+      locate(y[pb + p], dyr, ny, cy, f);
+      key = cx * ny + cy;
+      keys[pb + p] = key;
+    } else {
+      key = keys_in[pb + p];
+    }
+    kmin = min(kmin, key);
+    kmax = max(kmax, key);
+  }
+  kmin = wave_min_i(kmin);
+  kmax = wave_max_i(kmax);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { red[w] = kmin; red[4 + w] = kmax; }
+  __syncthreads();
+  kmin = min(min(red[0], red[1]), min(red[2], red[3]));
+  kmax = max(max(red[4], red[5]), max(red[6], red[7]));
+  const int* kv = FROM_POS ? keys : keys_in;
+  const long long span = (long long)kmax - kmin + 1;
+  if (span <= HWIN) {
+    for (int k = threadIdx.x; k < span; k += PIC_WG) hist[k] = 0;
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG)
+      rank[pb + p] = atomicAdd(&hist[kv[pb + p] - kmin], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < span; k += PIC_WG) {
+      int c = hist[k];
+      if (c) hist[k] = atomicAdd(&cnt[kmin + k], c);
+    }
+    __syncthreads();
+    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) rank[pb + p] += hist[kv[pb + p] - kmin];
+  } else {
+    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG)
+      rank[pb + p] = atomicAdd(&cnt[kv[pb + p]], 1);
+  }
+}
+
+// exclusive scan of the counts of one chunk of PIC_CELL_CHUNK cells
+__global__ __launch_bounds__(PIC_WG) void k_scan_local(const RunDesc* __restrict__ runs,
+                                                       const int* __restrict__ counts,
+                                                       int* __restrict__ offsets,
+                                                       int* __restrict__ chunk_sums,
+                                                       const int2* __restrict__ items) {
+  __shared__ int wsum[4];
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int G = rd.nx * rd.ny;
+  const int c0 = it.y + threadIdx.x * 8;
+  int v[8], s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int c = c0 + k;
+    v[k] = (c < G) ? counts[rd.cbase + c] : 0;
+    s += v[k];
+  }
+  // inclusive wave scan of per-thread totals
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int wpre = 0;
+  for (int k = 0; k < w; ++k) wpre += wsum[k];
+  int ex = wpre + inc - s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int c = c0 + k;
+    if (c < G) offsets[rd.cbase + c] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
+}
+
+// one workgroup per run: exclusive scan of the chunk sums, total -> offsets[G], nsorted
+__global__ __launch_bounds__(PIC_WG) void k_scan_chunks(const RunDesc* __restrict__ runs,
+                                                        RunState* __restrict__ st,
+                                                        int* __restrict__ chunk_sums,
+                                                        int* __restrict__ offsets) {
+  const RunDesc rd = runs[blockIdx.x];
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = rd.cell_item_begin; i < rd.cell_item_end; ++i) {
+      int v = chunk_sums[i];
+      chunk_sums[i] = acc;
+      acc += v;
+    }
+    offsets[rd.cbase + rd.nx * rd.ny] = acc;
+    st[blockIdx.x].nsorted = acc;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_scan_add(const RunDesc* __restrict__ runs,
+                                                     int* __restrict__ offsets,
+                                                     const int* __restrict__ chunk_sums,
+                                                     const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int G = rd.nx * rd.ny;
+  const int add = chunk_sums[blockIdx.x];
+  for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
+    int c = it.y + k;
+    if (c < G) offsets[rd.cbase + c] += add;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_sort_scatter(const RunDesc* __restrict__ runs,
+                                                         const RunState* __restrict__ st,
+                                                         const int* __restrict__ keys,
+                                                         const int* __restrict__ rank,
+                                                         const int* __restrict__ offsets,
+                                                         int* __restrict__ perm,
+                                                         const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int np = st[it.x].np;
+  const int p1 = min(it.y + PIC_CHUNK, np);
+  for (int p = it.y + threadIdx.x; p < p1; p += PIC_WG) {
+    int d = offsets[rd.cbase + keys[rd.pbase + p]] + rank[rd.pbase + p];
+    perm[rd.pbase + d] = p;
+  }
+}
+
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl_xor(lo, m, 64);
+  hi = __shfl_xor(hi, m, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long seg_key(const int* id, long long pb, int slot) {
+  unsigned k = id ? (unsigned)id[pb + slot] : (unsigned)slot;
+  return ((unsigned long long)k << 32) | (unsigned)slot;
+}
+
+// Canonical order inside each cell: ascending id.  Cells of <= 64 particles are sorted
+// by one wavefront in registers; larger cells are queued for k_segsort_big.
+__global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restrict__ runs,
+                                                          const int* __restrict__ offsets,
+                                                          int* __restrict__ perm,
+                                                          const int* __restrict__ id,
+                                                          int* __restrict__ big_list,
+                                                          int* __restrict__ big_count,
+                                                          const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int G = rd.nx * rd.ny;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cend = min(it.y + PIC_CELL_CHUNK, G);
+  const long long pb = rd.pbase;
+  for (int c = it.y + w; c < cend; c += 4) {
+    const int b = offsets[rd.cbase + c], e = offsets[rd.cbase + c + 1];
+    const int n = e - b;
+    if (n <= 1) continue;
+    if (n > 64) {
+      if (lane == 0) {
+        int slot = atomicAdd(big_count, 1);
+        big_list[2 * slot] = it.x;
+        big_list[2 * slot + 1] = c;
+      }
+      continue;
+    }
+    unsigned long long v = (lane < n) ? seg_key(id, pb, perm[pb + b + lane]) : ~0ull;
+    for (int k = 2; k <= 64; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        unsigned long long o = shfl_xor_u64(v, j);
+        bool up = (lane & k) == 0;
+        bool lower = (lane & j) == 0;
+        unsigned long long mn = v < o ? v : o, mx = v < o ? o : v;
+        v = (lower == up) ? mn : mx;
+      }
+    }
+    if (lane < n) perm[pb + b + lane] = (int)(unsigned)(v & 0xffffffffull);
+  }
+}
+
+// Large cells: bitonic sort by one workgroup, in LDS up to 4096 entries, else in a
+// global scratch region [2*(pbase+b), 2*(pbase+b) + m) with m = next pow2 >= n (<= 2n).
+__global__ __launch_bounds__(PIC_WG) void k_segsort_big(const RunDesc* __restrict__ runs,
+                                                        const int* __restrict__ offsets,
+                                                        int* __restrict__ perm,
+                                                        const int* __restrict__ id,
+                                                        const int* __restrict__ big_list,
+                                                        const int* __restrict__ big_count,
+                                                        unsigned long long* __restrict__ scratch) {
+  __shared__ unsigned long long s[4096];
+  const int nbig = *big_count;
+  for (int q = blockIdx.x; q < nbig; q += gridDim.x) {
+    const int run = big_list[2 * q], c = big_list[2 * q + 1];
+    const RunDesc rd = runs[run];
+    const long long pb = rd.pbase;
+    const int b = offsets[rd.cbase + c], e = offsets[rd.cbase + c + 1];
+    const int n = e - b;
+    int m = 1;
+    while (m < n) m <<= 1;
+    unsigned long long* a = (m <= 4096) ? s : scratch + 2 * (pb + b);
+    for (int i = threadIdx.x; i < m; i += PIC_WG)
+      a[i] = (i < n) ? seg_key(id, pb, perm[pb + b + i]) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < m; i += PIC_WG) {
+          int l = i ^ j;
+          if (l > i) {
+            unsigned long long u = a[i], v = a[l];
+            bool up = (i & k) == 0;
+            if ((u > v) == up) { a[i] = v; a[l] = u; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < n; i += PIC_WG)
+      perm[pb + b + i] = (int)(unsigned)(a[i] & 0xffffffffull);
+    __syncthreads();
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict__ runs,
+                                                       const RunState* __restrict__ st,
+                                                       const int* __restrict__ perm,
+                                                       PartBufs<R> src, PartBufs<R> dst,
+                                                       const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int np = st[it.x].np;
+  const int p1 = min(it.y + PIC_CHUNK, np);
+  const long long pb = rd.pbase;
+  for (int d = it.y + threadIdx.x; d < p1; d += PIC_WG) {
+    const long long s = pb + perm[pb + d];
+    dst.x[pb + d] = src.x[s];
+    dst.y[pb + d] = src.y[s];
+    dst.vx[pb + d] = src.vx[s];
+    dst.vy[pb + d] = src.vy[s];
+    dst.id[pb + d] = src.id[s];
+  }
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v) {
+  for (long long i = blockIdx.x * (long long)PIC_WG + threadIdx.x; i < n;
+       i += (long long)gridDim.x * PIC_WG)
+    p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, long long n) {
+  for (long long i = blockIdx.x * (long long)PIC_WG + threadIdx.x; i < n;
+       i += (long long)gridDim.x * PIC_WG)
+    p[i] = 0ull;
+}
+
+// ===================================================================== push
+// One workgroup per (tile, split): the particles of the tile are the T column ranges
+// [off[cx*ny + ty*T], off[cx*ny + ty*T + T]) of the cell-sorted arrays.  The new charge
+// is accumulated in an LDS window of (T+2H+1)^2 nodes around the tile with int64 atomics
+// and flushed once per workgroup; particles that drifted outside the window since the
+// last sort deposit straight to HBM.  Gather reads the padded E array (L2-resident).
+template <typename R>
+struct Geo {
+  int nx, ny;
+  R dx, dy, Lx, Ly;
+};
+
+template <typename R>
+__device__ __forceinline__ void deposit_weights(R xr, R yr, const Geo<R>& g, int& cx, int& cy,
+                                                long long w[4]) {
+  R fx, fy;
+  locate(xr, g.dx, g.nx, cx, fx);
+  locate(yr, g.dy, g.ny, cy, fy);
+  const long long one = 1ll << Fix<R>::QB;
+  long long ix1 = qweight(fx), iy1 = qweight(fy);
+  long long ix0 = one - ix1, iy0 = one - iy1;
+  w[0] = ix0 * iy0;
+  w[1] = ix1 * iy0;
+  w[2] = ix0 * iy1;
+  w[3] = ix1 * iy1;
+}
+
+template <typename R>
+__device__ __forceinline__ void deposit_global(unsigned long long* rho, unsigned long long* rlo,
+                                               int cx, int cy, const long long w[4],
+                                               const Geo<R>& g) {
+  const int cxp = (cx + 1 == g.nx) ? 0 : cx + 1, cyp = (cy + 1 == g.ny) ? 0 : cy + 1;
+  const long long nd[4] = {(long long)cx * g.ny + cy, (long long)cxp * g.ny + cy,
+                           (long long)cx * g.ny + cyp, (long long)cxp * g.ny + cyp};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (sizeof(R) == 8) {
+      atomicAdd(&rho[nd[k]], (unsigned long long)(w[k] >> 30));
+      atomicAdd(&rlo[nd[k]], (unsigned long long)(w[k] & ((1ll << 30) - 1)));
+    } else {
+      atomicAdd(&rho[nd[k]], (unsigned long long)w[k]);
+    }
+  }
+}
+
+// gather + kick + drift of one particle; returns synchronous-velocity sums
+template <typename R>
+__device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* __restrict__ E,
+                                         const Geo<R>& g, R hk, R dtr, double skc, double& ke,
+                                         double& px, double& py) {
+  int cx, cy;
+  R fx, fy;
+  locate(xr, g.dx, g.nx, cx, fx);
+  locate(yr, g.dy, g.ny, cy, fy);
+  const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
+  const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
+  const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
+  const R g00 = gx0 * gy0, g10 = gx1 * gy0, g01 = gx0 * gy1, g11 = gx1 * gy1;
+  typedef typename std::conditional<sizeof(R) == 4, float2, double2>::type R2;
+  const R2* e = reinterpret_cast<const R2*>(E);
+  const long long i00 = (long long)cx * (g.ny + 1) + cy;
+  const R2 e00 = e[i00], e01 = e[i00 + 1], e10 = e[i00 + g.ny + 1], e11 = e[i00 + g.ny + 2];
+  R epx = g00 * e00.x;
+  epx = epx + g10 * e10.x;
+  epx = epx + g01 * e01.x;
+  epx = epx + g11 * e11.x;
+  R epy = g00 * e00.y;
+  epy = epy + g10 * e10.y;
+  epy = epy + g01 * e01.y;
+  epy = epy + g11 * e11.y;
+  const double vsx = (double)vxr + skc * (double)epx, vsy = (double)vyr + skc * (double)epy;
+  ke += vsx * vsx + vsy * vsy;
+  px += vsx;
+  py += vsy;
+  vxr = vxr + hk * epx;
+  vyr = vyr + hk * epy;
+  xr = wrap_pos((R)(xr + dtr * vxr), g.Lx);
+  yr = wrap_pos((R)(yr + dtr * vyr), g.Ly);
+}
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(PIC_WG) void k_push(PushArgs<R> a) {
+  extern __shared__ unsigned long long win[];
+  __shared__ int colstart[65];
+  __shared__ int colbase[64];
+  __shared__ double red[4];
+  __shared__ unsigned int ticket_s;
+  const int4 it = a.items[blockIdx.x];
+  const int run = it.x;
+  const RunDesc rd = a.runs[run];
+  const RunState rs = a.st[run];
+  Geo<R> g;
+  g.nx = rd.nx; g.ny = rd.ny;
+  g.dx = (R)rd.dx; g.dy = (R)rd.dy; g.Lx = (R)rd.lx; g.Ly = (R)rd.ly;
+  const long long pb = rd.pbase;
+  R* __restrict__ X = a.x + pb;
+  R* __restrict__ Y = a.y + pb;
+  R* __restrict__ VX = a.vx + pb;
+  R* __restrict__ VY = a.vy + pb;
+  const R* __restrict__ E = a.E + 2 * rd.ebase;
+  const int par = (MODE == 0) ? (int)((rs.step + 1) & 1) : (int)(rs.step & 1);
+  unsigned long long* rho = a.rho + par * a.gsum + rd.gbase;
+  unsigned long long* rlo = (sizeof(R) == 8) ? a.rho_lo + par * a.gsum + rd.gbase : nullptr;
+  const double c = rs.vsync ? 0.5 : 1.0;
+  const R hk = (R)(c * rd.dt * rd.qm), dtr = (R)rd.dt;
+  const double skc = (c - 0.5) * rd.dt * rd.qm;
+  double ke = 0.0, px = 0.0, py = 0.0;
+
+  if (it.y >= 0) {
+    const int T = rd.tile, H = rd.halo, W = rd.win, S = rd.splits;
+    const int nwin = W * W * ((sizeof(R) == 8) ? 2 : 1);
+    for (int i = threadIdx.x; i < nwin; i += PIC_WG) win[i] = 0ull;
+    const int tx = it.y, ty = it.z, split = it.w;
+    if (threadIdx.x < T) {
+      const long long cb = rd.cbase + (long long)(tx * T + threadIdx.x) * g.ny + ty * T;
+      const int s0 = a.off[cb], s1 = a.off[cb + T];
+      colbase[threadIdx.x] = s0;
+      colstart[threadIdx.x + 1] = s1 - s0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      colstart[0] = 0;
+      for (int i = 0; i < T; ++i) colstart[i + 1] += colstart[i];
+    }
+    __syncthreads();
+    const long long F = colstart[T];
+    const int f0 = (int)(F * split / S), f1 = (int)(F * (split + 1) / S);
+    const int x0 = tx * T - H, y0 = ty * T - H;
+    unsigned long long* wlo = win + W * W;
+    for (int f = f0 + threadIdx.x; f < f1; f += PIC_WG) {
+      int lo = 0, hi = T - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (colstart[mid] <= f) lo = mid; else hi = mid - 1;
+      }
+      const int p = colbase[lo] + (f - colstart[lo]);
+      R xr = X[p], yr = Y[p];
+      if (MODE == 0) {
+        R vxr = VX[p], vyr = VY[p];
+        push_one<R>(xr, yr, vxr, vyr, E, g, hk, dtr, skc, ke, px, py);
+        X[p] = xr; Y[p] = yr; VX[p] = vxr; VY[p] = vyr;
+      }
+      int cx, cy;
+      long long w[4];
+      deposit_weights<R>(xr, yr, g, cx, cy, w);
+      int rx = cx - x0, ry = cy - y0;
+      rx += (rx < 0) ? g.nx : ((rx >= g.nx) ? -g.nx : 0);
+      ry += (ry < 0) ? g.ny : ((ry >= g.ny) ? -g.ny : 0);
+      if (rx < W - 1 && ry < W - 1) {
+        const int i0 = rx * W + ry;
+        if (sizeof(R) == 8) {
+          const long long m = (1ll << 30) - 1;
+          atomicAdd(&win[i0], (unsigned long long)(w[0] >> 30));
+          atomicAdd(&win[i0 + W], (unsigned long long)(w[1] >> 30));
+          atomicAdd(&win[i0 + 1], (unsigned long long)(w[2] >> 30));
+          atomicAdd(&win[i0 + W + 1], (unsigned long long)(w[3] >> 30));
+          atomicAdd(&wlo[i0], (unsigned long long)(w[0] & m));
+          atomicAdd(&wlo[i0 + W], (unsigned long long)(w[1] & m));
+          atomicAdd(&wlo[i0 + 1], (unsigned long long)(w[2] & m));
+          atomicAdd(&wlo[i0 + W + 1], (unsigned long long)(w[3] & m));
+        } else {
+          atomicAdd(&win[i0], (unsigned long long)w[0]);
+          atomicAdd(&win[i0 + W], (unsigned long long)w[1]);
+          atomicAdd(&win[i0 + 1], (unsigned long long)w[2]);
+          atomicAdd(&win[i0 + W + 1], (unsigned long long)w[3]);
+        }
+      } else {
+        deposit_global<R>(rho, rlo, cx, cy, w, g);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < W * W; i += PIC_WG) {
+      const unsigned long long v = win[i];
+      const unsigned long long vl = (sizeof(R) == 8) ? wlo[i] : 0ull;
+      if (v | vl) {
+        int gx = (x0 + i / W) % g.nx, gy = (y0 + i % W) % g.ny;
+        gx += (gx < 0) ? g.nx : 0;
+        gy += (gy < 0) ? g.ny : 0;
+        const long long nd = (long long)gx * g.ny + gy;
+        if (v) atomicAdd(&rho[nd], v);
+        if (sizeof(R) == 8 && vl) atomicAdd(&rlo[nd], vl);
+      }
+    }
+  } else {
+    // tail chunk: particles appended after the last sort (not covered by the offsets)
+    const int p0 = max(it.z, rs.nsorted), p1 = min(it.z + PIC_CHUNK, rs.np);
+    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) {
+      R xr = X[p], yr = Y[p];
+      if (MODE == 0) {
+        R vxr = VX[p], vyr = VY[p];
+        push_one<R>(xr, yr, vxr, vyr, E, g, hk, dtr, skc, ke, px, py);
+        X[p] = xr; Y[p] = yr; VX[p] = vxr; VY[p] = vyr;
+      }
+      int cx, cy;
+      long long w[4];
+      deposit_weights<R>(xr, yr, g, cx, cy, w);
+      deposit_global<R>(rho, rlo, cx, cy, w, g);
+    }
+  }
+  if (MODE != 0) return;
+
+  // per-workgroup partial sums, then the last workgroup reduces them in item order
+  const double ske = block_sum(ke, red);
+  const double spx = block_sum(px, red);
+  const double spy = block_sum(py, red);
+  if (threadIdx.x == 0) {
+    a.partials[3 * (long long)blockIdx.x + 0] = ske;
+    a.partials[3 * (long long)blockIdx.x + 1] = spx;
+    a.partials[3 * (long long)blockIdx.x + 2] = spy;
+    __threadfence();
+    ticket_s = atomicAdd(a.ticket, 1u);
+  }
+  __syncthreads();
+  if (ticket_s != gridDim.x - 1) return;
+  __threadfence();
+  for (int r = 0; r < a.nruns; ++r) {
+    const RunDesc d = a.runs[r];
+    double s3[3];
+    for (int k = 0; k < 3; ++k) {
+      double v = 0.0;
+      for (int i = d.push_item_begin + threadIdx.x; i < d.push_item_end; i += PIC_WG)
+        v += a.partials[3 * (long long)i + k];
+      s3[k] = block_sum(v, red);
+    }
+    if (threadIdx.x == 0) {
+      RunState* sr = a.st + r;
+      const long long stp = sr->step;
+      const double mp = d.qp / d.qm;
+      double* row = a.diag + (d.dbase + (stp % PIC_DIAG_CAP)) * 8;
+      row[1] = 0.5 * mp * s3[0];
+      row[2] = mp * s3[1];
+      row[3] = mp * s3[2];
+      row[5] = (double)sr->np;
+      sr->step = stp + 1;
+      sr->vsync = 0;
+    }
+  }
+  if (threadIdx.x == 0) *a.ticket = 0u;
+}
+
+// synchronous velocities for output: v + 1/2 dt qm E(x) when a half kick is pending
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_half_kick_out(const RunDesc* __restrict__ runs,
+                                                          const RunState* __restrict__ st,
+                                                          int run, PartBufs<R> p,
+                                                          const R* __restrict__ Eall,
+                                                          R* __restrict__ ovx,
+                                                          R* __restrict__ ovy,
+                                                          const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  if (it.x != run) return;
+  const RunDesc rd = runs[run];
+  const RunState rs = st[run];
+  Geo<R> g;
+  g.nx = rd.nx; g.ny = rd.ny;
+  g.dx = (R)rd.dx; g.dy = (R)rd.dy; g.Lx = (R)rd.lx; g.Ly = (R)rd.ly;
+  const R* E = Eall + 2 * rd.ebase;
+  const R hk = rs.vsync ? (R)0 : (R)(0.5 * rd.dt * rd.qm);
+  const int p1 = min(it.y + PIC_CHUNK, rs.np);
+  for (int q = it.y + threadIdx.x; q < p1; q += PIC_WG) {
+    const long long i = rd.pbase + q;
+    R xr = p.x[i], yr = p.y[i], vxr = p.vx[i], vyr = p.vy[i];
+    double ke = 0, px = 0, py = 0;
+    R x2 = xr, y2 = yr;
+    push_one<R>(x2, y2, vxr, vyr, E, g, hk, (R)0, 0.0, ke, px, py);
+    ovx[q] = vxr;
+    ovy[q] = vyr;
+  }
+}
+
+// scatter slot q of run `run` to output index id[q] (id order)
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_unpermute(const RunDesc* __restrict__ runs,
+                                                      const RunState* __restrict__ st, int run,
+                                                      PartBufs<R> p, const R* __restrict__ vxs,
+                                                      const R* __restrict__ vys,
+                                                      R* __restrict__ ox, R* __restrict__ oy,
+                                                      R* __restrict__ ovx, R* __restrict__ ovy,
+                                                      const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  if (it.x != run) return;
+  const RunDesc rd = runs[run];
+  const int np = st[run].np;
+  const int p1 = min(it.y + PIC_CHUNK, np);
+  for (int q = it.y + threadIdx.x; q < p1; q += PIC_WG) {
+    const long long i = rd.pbase + q;
+    const int k = p.id[i];
+    if (k < 0 || k >= np) continue;
+    ox[k] = p.x[i];
+    oy[k] = p.y[i];
+    ovx[k] = vxs ? vxs[q] : p.vx[i];
+    ovy[k] = vys ? vys[q] : p.vy[i];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_check_finite(const RunDesc* __restrict__ runs,
+                                                         const RunState* __restrict__ st,
+                                                         PartBufs<R> p,
+                                                         const int2* __restrict__ items,
+                                                         int* flag) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int p1 = min(it.y + PIC_CHUNK, st[it.x].np);
+  bool bad = false;
+  for (int q = it.y + threadIdx.x; q < p1; q += PIC_WG) {
+    const long long i = rd.pbase + q;
+    R s = p.x[i] + p.y[i] + p.vx[i] + p.vy[i];
+    if (!isfinite(s)) bad = true;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+// ===================================================================== ops
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_op_cell_keys(const R* __restrict__ x,
+                                                         const R* __restrict__ y, int64_t n,
+                                                         int nx, int ny, double dx, double dy,
+                                                         int* __restrict__ keys) {
+  for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * PIC_WG) {
+    int cx, cy;
+    R f;
+    locate(x[p], (R)dx, nx, cx, f);
+    locate(y[p], (R)dy, ny, cy, f);
+    keys[p] = cx * ny + cy;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_op_gather_push(R* x, R* y, R* vx, R* vy, int64_t n,
+                                                           const double* __restrict__ ex,
+                                                           const double* __restrict__ ey,
+                                                           int nx, int ny, double lx, double ly,
+                                                           double dt, double qm, double c,
+                                                           double* partials) {
+  __shared__ double red[4];
+  Geo<R> g;
+  g.nx = nx; g.ny = ny;
+  g.dx = (R)(lx / nx); g.dy = (R)(ly / ny); g.Lx = (R)lx; g.Ly = (R)ly;
+  const R hk = (R)(c * dt * qm), dtr = (R)dt;
+  const double skc = (c - 0.5) * dt * qm;
+  double ke = 0, px = 0, py = 0;
+  for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * PIC_WG) {
+    R xr = x[p], yr = y[p], vxr = vx[p], vyr = vy[p];
+    int cx, cy;
+    R fx, fy;
+    locate(xr, g.dx, nx, cx, fx);
+    locate(yr, g.dy, ny, cy, fy);
+    const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
+    const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
+    const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
+    const R g00 = gx0 * gy0, g10 = gx1 * gy0, g01 = gx0 * gy1, g11 = gx1 * gy1;
+    const int cxp = (cx + 1 == nx) ? 0 : cx + 1, cyp = (cy + 1 == ny) ? 0 : cy + 1;
+    const long long n00 = (long long)cx * ny + cy, n10 = (long long)cxp * ny + cy,
+                    n01 = (long long)cx * ny + cyp, n11 = (long long)cxp * ny + cyp;
+    R epx = g00 * (R)ex[n00];
+    epx = epx + g10 * (R)ex[n10];
+    epx = epx + g01 * (R)ex[n01];
+    epx = epx + g11 * (R)ex[n11];
+    R epy = g00 * (R)ey[n00];
+    epy = epy + g10 * (R)ey[n10];
+    epy = epy + g01 * (R)ey[n01];
+    epy = epy + g11 * (R)ey[n11];
+    const double vsx = (double)vxr + skc * (double)epx, vsy = (double)vyr + skc * (double)epy;
+    ke += vsx * vsx + vsy * vsy;
+    vxr = vxr + hk * epx;
+    vyr = vyr + hk * epy;
+    x[p] = wrap_pos((R)(xr + dtr * vxr), g.Lx);
+    y[p] = wrap_pos((R)(yr + dtr * vyr), g.Ly);
+    vx[p] = vxr;
+    vy[p] = vyr;
+  }
+  const double s = block_sum(ke, red);
+  if (partials && threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_op_deposit(const R* __restrict__ x,
+                                                       const R* __restrict__ y, int64_t n,
+                                                       int nx, int ny, double lx, double ly,
+                                                       unsigned long long* hi,
+                                                       unsigned long long* lo) {
+  Geo<R> g;
+  g.nx = nx; g.ny = ny;
+  g.dx = (R)(lx / nx); g.dy = (R)(ly / ny); g.Lx = (R)lx; g.Ly = (R)ly;
+  for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * PIC_WG) {
+    int cx, cy;
+    long long w[4];
+    deposit_weights<R>(x[p], y[p], g, cx, cy, w);
+    deposit_global<R>(hi, lo, cx, cy, w, g);
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_op_fixed_to_double(const unsigned long long* hi,
+                                                               const unsigned long long* lo,
+                                                               long long G, int fp64,
+                                                               double scale, double* out) {
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
+       q += (long long)gridDim.x * PIC_WG) {
+    if (fp64)
+      out[q] = ((double)(long long)hi[q] * 9.313225746154785e-10 +
+                (double)(long long)lo[q] * 8.673617379884035e-19) * scale;
+    else
+      out[q] = (double)(long long)hi[q] * scale;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_op_validate_keys(const int* keys, int64_t n,
+                                                             int ncells, int* flag) {
+  bool bad = false;
+  for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * PIC_WG)
+    if (keys[p] < 0 || keys[p] >= ncells) bad = true;
+  if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_reduce_sum(const double* parts, int n, double* out) {
+  __shared__ double red[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += PIC_WG) v += parts[i];
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) *out = v;
+}
+
+inline int grid_for(long long n) {
+  long long b = (n + PIC_WG - 1) / PIC_WG;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+}  // namespace
+
+// ====================================================================== launchers
+void launch_fill_i32(hipStream_t s, int* p, long long n, int v) {
+  if (n > 0) k_fill_i32<<<grid_for(n), PIC_WG, 0, s>>>(p, n, v);
+}
+void launch_fill_u64(hipStream_t s, unsigned long long* p, long long n) {
+  if (n > 0) k_fill_u64<<<grid_for(n), PIC_WG, 0, s>>>(p, n);
+}
+template <typename R>
+void launch_op_deposit(hipStream_t s, const R* x, const R* y, int64_t n, int nx, int ny,
+                       double lx, double ly, unsigned long long* hi, unsigned long long* lo) {
+  k_op_deposit<R><<<grid_for(n), PIC_WG, 0, s>>>(x, y, n, nx, ny, lx, ly, hi, lo);
+}
+template void launch_op_deposit<float>(hipStream_t, const float*, const float*, int64_t, int, int,
+                                       double, double, unsigned long long*, unsigned long long*);
+template void launch_op_deposit<double>(hipStream_t, const double*, const double*, int64_t, int,
+                                        int, double, double, unsigned long long*,
+                                        unsigned long long*);
+void launch_op_fixed_to_double(hipStream_t s, const unsigned long long* hi,
+                               const unsigned long long* lo, long long G, int fp64, double scale,
+                               double* out) {
+  k_op_fixed_to_double<<<grid_for(G), PIC_WG, 0, s>>>(hi, lo, G, fp64, scale, out);
+}
+void launch_op_validate_keys(hipStream_t s, const int* keys, int64_t n, int ncells, int* flag) {
+  k_op_validate_keys<<<grid_for(n), PIC_WG, 0, s>>>(keys, n, ncells, flag);
+}
+void launch_reduce_sum(hipStream_t s, const double* parts, int n, double* out) {
+  k_reduce_sum<<<1, PIC_WG, 0, s>>>(parts, n, out);
+}
+
+static void sort_tail(hipStream_t s, const RunDesc* runs, RunState* st, const int* id,
+                      SortBufs sb, const int2* citems, int ncitems) {
+  launch_fill_i32(s, sb.big_count, 1, 0);
+  k_segsort_small<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list,
+                                             sb.big_count, citems);
+  k_segsort_big<<<256, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list, sb.big_count,
+                                       sb.scratch);
+}
+
+static void sort_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, SortBufs sb,
+                      const int2* citems, int ncitems) {
+  k_scan_local<<<ncitems, PIC_WG, 0, s>>>(runs, sb.counts, sb.offsets, sb.chunk_sums, citems);
+  k_scan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.chunk_sums, sb.offsets);
+  k_scan_add<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.chunk_sums, citems);
+}
+
+template <typename R>
+void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, PartBufs<R> src,
+                 PartBufs<R> dst, SortBufs sb, const int2* pitems, int npitems,
+                 const int2* citems, int ncitems, long long ctotal, bool move_data) {
+  launch_fill_i32(s, sb.counts, ctotal, 0);
+  k_sort_rank<R, true><<<npitems, PIC_WG, 0, s>>>(runs, st, src.x, src.y, nullptr, sb.keys,
+                                                 sb.rank, sb.counts, pitems);
+  sort_scan(s, runs, st, nruns, sb, citems, ncitems);
+  k_sort_scatter<<<npitems, PIC_WG, 0, s>>>(runs, st, sb.keys, sb.rank, sb.offsets, sb.perm,
+                                            pitems);
+  sort_tail(s, runs, st, src.id, sb, citems, ncitems);
+  if (move_data)
+    k_sort_apply<R><<<npitems, PIC_WG, 0, s>>>(runs, st, sb.perm, src, dst, pitems);
+}
+
+void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,
+                           SortBufs sb, const int2* pitems, int npitems, const int2* citems,
+                           int ncitems, long long ctotal) {
+  launch_fill_i32(s, sb.counts, ctotal, 0);
+  k_sort_rank<float, false><<<npitems, PIC_WG, 0, s>>>(runs, st, nullptr, nullptr, keys,
+                                                      sb.keys, sb.rank, sb.counts, pitems);
+  sort_scan(s, runs, st, 1, sb, citems, ncitems);
+  k_sort_scatter<<<npitems, PIC_WG, 0, s>>>(runs, st, keys, sb.rank, sb.offsets, sb.perm, pitems);
+  sort_tail(s, runs, st, nullptr, sb, citems, ncitems);
+}
+
+template <typename R>
+void launch_push(hipStream_t s, const PushArgs<R>& a) {
+  if (a.nitems <= 0) return;
+  if (a.mode == 0)
+    k_push<R, 0><<<a.nitems, PIC_WG, a.lds_bytes, s>>>(a);
+  else
+    k_push<R, 1><<<a.nitems, PIC_WG, a.lds_bytes, s>>>(a);
+}
+
+template <typename R>
+void launch_half_kick_out(hipStream_t s, const RunDesc* runs, RunState* st, int run,
+                          PartBufs<R> p, const R* E, R* out_vx, R* out_vy, const int2* pitems,
+                          int npitems) {
+  k_half_kick_out<R><<<npitems, PIC_WG, 0, s>>>(runs, st, run, p, E, out_vx, out_vy, pitems);
+}
+
+template <typename R>
+void launch_unpermute(hipStream_t s, const RunDesc* runs, RunState* st, int run, PartBufs<R> p,
+                      const R* vxs, const R* vys, R* ox, R* oy, R* ovx, R* ovy,
+                      const int2* pitems, int npitems) {
+  k_unpermute<R><<<npitems, PIC_WG, 0, s>>>(runs, st, run, p, vxs, vys, ox, oy, ovx, ovy, pitems);
+}
+
+template <typename R>
+void launch_check_finite(hipStream_t s, const RunDesc* runs, RunState* st, PartBufs<R> p,
+                         const int2* pitems, int npitems, int* flag) {
+  k_check_finite<R><<<npitems, PIC_WG, 0, s>>>(runs, st, p, pitems, flag);
+}
+
+template <typename R>
+void launch_op_cell_keys(hipStream_t s, const R* x, const R* y, int64_t n, int nx, int ny,
+                         double lx, double ly, int* keys) {
+  k_op_cell_keys<R><<<grid_for(n), PIC_WG, 0, s>>>(x, y, n, nx, ny, lx / nx, ly / ny, keys);
+}
+
+template <typename R>
+void launch_op_gather_push(hipStream_t s, R* x, R* y, R* vx, R* vy, int64_t n, const double* ex,
+                           const double* ey, int nx, int ny, double lx, double ly, double dt,
+                           double qm, double c, double* ke_partials, int nblocks) {
+  k_op_gather_push<R><<<nblocks, PIC_WG, 0, s>>>(x, y, vx, vy, n, ex, ey, nx, ny, lx, ly, dt, qm,
+                                                 c, ke_partials);
+}
+
+#define PIC_INST(R)                                                                          \
+  template void launch_sort<R>(hipStream_t, const RunDesc*, RunState*, int, PartBufs<R>,    \
+                               PartBufs<R>, SortBufs, const int2*, int, const int2*, int,   \
+                               long long, bool);                                            \
+  template void launch_push<R>(hipStream_t, const PushArgs<R>&);                            \
+  template void launch_half_kick_out<R>(hipStream_t, const RunDesc*, RunState*, int,        \
+                                        PartBufs<R>, const R*, R*, R*, const int2*, int);   \
+  template void launch_unpermute<R>(hipStream_t, const RunDesc*, RunState*, int,            \
+                                    PartBufs<R>, const R*, const R*, R*, R*, R*, R*,        \
+                                    const int2*, int);                                      \
+  template void launch_check_finite<R>(hipStream_t, const RunDesc*, RunState*, PartBufs<R>, \
+                                       const int2*, int, int*);                             \
+  template void launch_op_cell_keys<R>(hipStream_t, const R*, const R*, int64_t, int, int,  \
+                                       double, double, int*);                               \
+  template void launch_op_gather_push<R>(hipStream_t, R*, R*, R*, R*, int64_t,              \
+                                         const double*, const double*, int, int, double,    \
+                                         double, double, double, double, double*, int);
+PIC_INST(float)
+PIC_INST(double)

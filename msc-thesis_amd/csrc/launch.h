@@ -1,0 +1,167 @@
+// launch.h -- host-side launchers of the HIP kernels (internal to libpicamd.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pic_device.h"
+
+// Item lists (device, built once per context):
+//   chunk items  int2 {run, start}: PIC_CHUNK particles or PIC_CELL_CHUNK cells
+//   tile items   int4 {run, tx, ty, split}; tx < 0 marks a tail chunk (start in .z)
+//   row items    int2 {run, row}
+#define PIC_CELL_CHUNK 2048
+
+struct SortBufs {
+  int* keys;        // [ptotal]
+  int* rank;        // [ptotal]
+  int* perm;        // [ptotal]  run-local source index of sorted slot
+  int* counts;      // [ctotal]
+  int* offsets;     // [ctotal]
+  int* chunk_sums;  // [n cell chunk items]
+  int* big_list;    // [ptotal] cells needing the workgroup segment sort (run, cell) pairs
+  int* big_count;   // [1]
+  unsigned long long* scratch;  // [2*ptotal] power-of-two padded segments
+};
+
+template <typename R>
+struct PartBufs {
+  R *x, *y, *vx, *vy;
+  int* id;
+};
+
+// ---- particle kernels (k_particles.hip)
+template <typename R>
+void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                 PartBufs<R> src, PartBufs<R> dst, SortBufs sb,
+                 const int2* pitems, int npitems, const int2* citems, int ncitems,
+                 long long ctotal, bool move_data);
+// op: keys given (run 0 layout), produce perm/counts/offsets
+void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,
+                           SortBufs sb, const int2* pitems, int npitems, const int2* citems,
+                           int ncitems, long long ctotal);
+
+template <typename R>
+struct PushArgs {
+  const RunDesc* runs;
+  RunState* st;
+  R *x, *y, *vx, *vy;
+  const R* E;
+  unsigned long long* rho;      // [2][gsum] (hi word for FP64)
+  unsigned long long* rho_lo;   // [2][gsum] (FP64 only)
+  long long gsum;
+  const int* off;
+  const int4* items;
+  int nitems;
+  int nruns;
+  double* partials;             // [nitems*3]
+  unsigned int* ticket;
+  double* diag;
+  int mode;                     // 0: gather+push+deposit, 1: deposit only
+  int lds_bytes;
+};
+template <typename R>
+void launch_push(hipStream_t s, const PushArgs<R>& a);
+
+template <typename R>
+void launch_half_kick_out(hipStream_t s, const RunDesc* runs, RunState* st, int run,
+                          PartBufs<R> p, const R* E, R* out_vx, R* out_vy,
+                          const int2* pitems, int npitems);
+template <typename R>
+void launch_unpermute(hipStream_t s, const RunDesc* runs, RunState* st, int run,
+                      PartBufs<R> p, const R* vxs, const R* vys, R* ox, R* oy, R* ovx, R* ovy,
+                      const int2* pitems, int npitems);
+template <typename R>
+void launch_check_finite(hipStream_t s, const RunDesc* runs, RunState* st, PartBufs<R> p,
+                         const int2* pitems, int npitems, int* flag);
+
+// ops on plain device arrays (run 0 descriptor)
+template <typename R>
+void launch_op_cell_keys(hipStream_t s, const R* x, const R* y, int64_t n, int nx, int ny,
+                         double lx, double ly, int* keys);
+template <typename R>
+void launch_op_gather_push(hipStream_t s, R* x, R* y, R* vx, R* vy, int64_t n, const double* ex,
+                           const double* ey, int nx, int ny, double lx, double ly, double dt,
+                           double qm, double c, double* ke_partials, int nblocks);
+template <typename R>
+void launch_op_deposit(hipStream_t s, const R* x, const R* y, int64_t n, int nx, int ny,
+                       double lx, double ly, unsigned long long* hi, unsigned long long* lo);
+void launch_op_fixed_to_double(hipStream_t s, const unsigned long long* hi,
+                               const unsigned long long* lo, long long G, int fp64,
+                               double scale, double* out);
+void launch_op_validate_keys(hipStream_t s, const int* keys, int64_t n, int ncells, int* flag);
+void launch_fill_i32(hipStream_t s, int* p, long long n, int v);
+void launch_fill_u64(hipStream_t s, unsigned long long* p, long long n);
+void launch_reduce_sum(hipStream_t s, const double* parts, int n, double* out);
+
+// ---- field kernels (k_field.hip)
+struct FieldArgs {
+  const RunDesc* runs;
+  RunState* st;
+  const unsigned long long* rho;     // [2][gsum]
+  unsigned long long* rho_w;         // same buffer, written (zeroing of the next parity)
+  const unsigned long long* rho_lo;  // FP64 only
+  unsigned long long* rho_lo_w;
+  long long gsum;
+  double2* S;    // rho_hat, then phi_hat
+  double2* Sx;   // Ex_hat
+  double2* Sy;   // Ey_hat
+  void* E;       // padded (Ex,Ey) pairs, float or double
+  double* phi;   // optional node potential (may be NULL)
+  const double2* tw;
+  const int2* row_pair_items; int n_row_pair_items;
+  const int2* col_items; int n_col_items;
+  const int2* row_items; int n_row_items;
+  double* partials;                  // [n_row_items]
+  unsigned int* ticket;
+  double* diag;
+  int nruns;
+  int fp64;                          // particle precision (E type, rho words)
+  int want_phi;
+  int max_nx, max_ny;                // largest grid of the context (LDS sizing)
+  const double* rho_in;              // op mode: FP64 density input instead of rho (may be NULL)
+};
+void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev /* 4 events or NULL */);
+void launch_op_efield(hipStream_t s, const double* phi, double* ex, double* ey, int nx, int ny,
+                      double lx, double ly);
+// copy padded E (float/double pairs) of one run out to two FP64 node arrays
+void launch_unpad_E(hipStream_t s, const void* E, int fp64, int nx, int ny, double* ex, double* ey);
+// rho (fixed point, parity of current step) of one run to FP64 density incl. background
+void launch_rho_to_double(hipStream_t s, const RunDesc* runs, RunState* st, int run,
+                          const unsigned long long* rho, const unsigned long long* rho_lo,
+                          long long gsum, int fp64, double background, double* out);
+
+// ---- collision kernels (k_collide.hip)
+struct MccArgs {
+  float *x, *y, *vx, *vy;
+  int* id;
+  RunState* st;       // np / next_id / err of the run (device)
+  long long pbase;    // particle base of the run
+  int cap;
+  double v_unit2_half_me_over_e;  // E[eV] = this * |v|^2
+  double ng_vunit_dtphys;         // nu*dt = this * sigma(E) * |v|
+  double p_coll;                  // 1 - exp(-nu_max dt)
+  double nu_max_dt;               // nu_max * dt_phys
+  double mass_ratio2;             // 2 m_e / M
+  uint64_t seed;
+  long long* counts;              // [4]
+  int* birth_count;
+  int* birth_parent;              // [cap]
+  float* birth_buf;               // [4*cap]
+  int from_state_step;            // 1: take the step index from st->step
+  long long step;
+};
+void launch_mcc(hipStream_t s, const MccArgs& a, int n_upper);
+void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scratch,
+                       unsigned long long* rho, long long gsum, const RunDesc* runs, int run);
+struct DsmcArgs {
+  float *vx, *vy;
+  const int* id;
+  const int* offsets;
+  float* gmax;
+  int ncells;
+  double coeff;
+  uint64_t seed;
+  long long step;
+  const RunState* st;             // may be NULL (op mode uses `step`)
+  long long* ncoll;
+};
+void launch_dsmc(hipStream_t s, const DsmcArgs& a);

@@ -1,0 +1,162 @@
+/*
+ * oracle.h -- CPU restatement of the reference's 2D electrostatic PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path (libpicamd.so) never links or calls it.
+ *
+ * Two flavours of every particle stage are provided:
+ *   - "f64": the reference algorithm (thesisCode/collisionless/double) with the
+ *     intended physics (periodic CIC, spectral Poisson, central-difference E,
+ *     leapfrog), continuous CIC weights, plain double accumulation;
+ *   - "mirror": the exact operation sequence the GPU kernels use (FP32 or FP64
+ *     particles, per-axis fixed-point CIC weights, int64 accumulation), so
+ *     integer outputs and fixed-point charge can be compared bit-for-bit.
+ *
+ * Pinning of each function (tests/test_oracle_*.py):
+ *   cell keys / stable sort ........ brute force std::stable_sort, bijection
+ *   deposit ........................ partition of unity, single-particle
+ *                                    stencils, reference grid.cc via oracle/_ref
+ *                                    (reflected-weight bridge)
+ *   FFT / Poisson .................. brute-force DFT, single-mode closed form,
+ *                                    numpy fixtures (tests/golden)
+ *   E-field ........................ closed form sin(k dx)/dx factor
+ *   gather / push .................. constant-field ballistic orbit, zero
+ *                                    self-force, reference particle.cc via _ref
+ *   whole step ..................... Landau damping rate, two-stream growth
+ *   Philox4x32-10 .................. Random123 known-answer vectors
+ *   std engines .................... ISO C++ [rand.predef] 10000th values
+ *   MCC / DSMC ..................... closed-form rates, conservation laws
+ *   adjoint ........................ dot-product tests, central FD
+ */
+#ifndef PIC_ORACLE_H
+#define PIC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- random numbers ---------------- */
+/* Philox4x32-10 (Salmon et al., SC'11). */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* n-th output (1-based) of std::minstd_rand0 / std::mt19937 seeded with `seed`. */
+uint64_t or_minstd_rand0_nth(uint32_t seed, int n);
+uint64_t or_mt19937_nth(uint32_t seed, int n);
+
+/* ---------------- loaders ---------------- */
+/* Reference lattice + Maxwellian loader: thesisCode/collisionless/double/src/main.cc:201-244
+ * (std::default_random_engine(1234) + std::normal_distribution<double>, vx then vy,
+ * row-major; integer-division offset kept as in the reference). */
+void or_init_reference(int npx, int npy, int xdim, int ydim, int nx, int ny,
+                       double vtx, double vty, double vdx, double vdy,
+                       double* x, double* y, double* vx, double* vy);
+/* Quiet-lattice loaders with Philox Box-Muller velocities (no reference counterpart). */
+void or_init_two_stream(int nx, int ny, double lx, double ly, int ppc, double v0,
+                        double vth, double amp, int mode, uint64_t seed,
+                        double* x, double* y, double* vx, double* vy);
+void or_init_landau(int nx, int ny, double lx, double ly, int ppc, double vth,
+                    double alpha, int mode, uint64_t seed,
+                    double* x, double* y, double* vx, double* vy);
+/* Lattice factorisation used by the loaders: npx = nx*a, npy = ny*b, a*b = ppc. */
+void or_lattice_shape(int nx, int ny, int ppc, int* npx, int* npy);
+
+/* ---------------- stages ---------------- */
+/* Cell key cx*ny+cy with cx=(int)(x/dx) (thesisCode/collisionless/double/src/pic.cu:69-87). */
+void or_cell_keys_f64(int64_t n, const double* x, const double* y, int nx, int ny,
+                      double lx, double ly, int32_t* keys);
+void or_cell_keys_f32(int64_t n, const float* x, const float* y, int nx, int ny,
+                      double lx, double ly, int32_t* keys);
+/* Stable counting sort by key (ascending particle index inside a cell),
+ * thesisCode/dsmcCollisions/cellTrackerAdvanced/src/pic.cu:419-445. */
+void or_stable_sort(int64_t n, const int32_t* keys, int ncells,
+                    int32_t* perm, int32_t* counts, int32_t* offsets);
+
+/* Continuous-weight CIC charge density (charge per area, FP64), no background.
+ * thesisCode/collisionless/double/src/grid.cc:46-76 with intended CIC weights. */
+void or_deposit_f64(int64_t n, const double* x, const double* y, int nx, int ny,
+                    double lx, double ly, double qp, double* rho);
+/* Fixed-point mirrors of the GPU deposit. f32: one int64 word, 2^18 per axis.
+ * f64: two int64 words (hi = W>>30, lo = W & (2^30-1)), 2^30 per axis. */
+void or_deposit_fixed_f32(int64_t n, const float* x, const float* y, int nx, int ny,
+                          double lx, double ly, int64_t* acc);
+void or_deposit_fixed_f64(int64_t n, const double* x, const double* y, int nx, int ny,
+                          double lx, double ly, int64_t* hi, int64_t* lo);
+/* Convert fixed-point accumulators to charge density (charge per area). */
+void or_fixed_to_rho_f32(int nx, int ny, double lx, double ly, double qp,
+                         const int64_t* acc, double* rho);
+void or_fixed_to_rho_f64(int nx, int ny, double lx, double ly, double qp,
+                         const int64_t* hi, const int64_t* lo, double* rho);
+
+/* 2D complex FFT helpers (radix-2, FP64), in place, sign=-1 forward, +1 inverse
+ * (unnormalised).  Data is row-major [nx][ny] interleaved (re,im). */
+void or_fft2(int nx, int ny, double* data, int sign);
+void or_dft2_brute(int nx, int ny, const double* in, double* out, int sign);
+
+/* Spectral Poisson: laplacian(phi) = -rho, k = 2*pi*m/L, k=0 -> 0
+ * (thesisCode/collisionless/double/src/fourier.cc:15-126 with intended k, sign). */
+void or_poisson(int nx, int ny, double lx, double ly, const double* rho, double* phi);
+/* E = -grad(phi), periodic central difference / (2 dx)
+ * (thesisCode/collisionless/double/src/grid.cc:85-96, axes as intended). */
+void or_efield(int nx, int ny, double lx, double ly, const double* phi,
+               double* ex, double* ey);
+
+/* Gather + leapfrog push.  kick = c*dt (c = 0.5 for the first step from synchronous
+ * velocities, 1 afterwards).  ex/ey are node fields [nx*ny].  quantized=1 uses the
+ * GPU's fixed-point gather weights and FP operation order (mirror); 0 uses
+ * continuous weights.  ke_out (may be NULL) receives sum of |v_n|^2 where v_n is
+ * the synchronous velocity at the start of the step.
+ * thesisCode/collisionless/double/src/particle.cc:37-106 (intended KDK). */
+void or_push_f64(int64_t n, double* x, double* y, double* vx, double* vy,
+                 const double* ex, const double* ey, int nx, int ny,
+                 double lx, double ly, double dt, double qm, double c, int quantized,
+                 double* ke_out, double* px_out, double* py_out);
+void or_push_f32(int64_t n, float* x, float* y, float* vx, float* vy,
+                 const float* ex, const float* ey, int nx, int ny,
+                 double lx, double ly, double dt, double qm, double c,
+                 double* ke_out, double* px_out, double* py_out);
+/* Interpolated field at particle positions (continuous weights, FP64). */
+void or_gather_f64(int64_t n, const double* x, const double* y, const double* ex,
+                   const double* ey, int nx, int ny, double lx, double ly,
+                   double* epx, double* epy);
+
+/* Whole collisionless run in FP64 (continuous weights).  diag (may be NULL) gets
+ * nsteps rows of {field_energy, kinetic_energy, px, py, |phi_hat(1,0)|}.
+ * Velocities on return are synchronous (final half kick applied).
+ * Step order: thesisCode/collisionless/double/src/main.cc:137-172. */
+void or_run_f64(int64_t n, double* x, double* y, double* vx, double* vy,
+                int nx, int ny, double lx, double ly, double dt, double qm,
+                double qp, int nsteps, double* diag);
+/* Same in the GPU mirror arithmetic (FP32 particles, fixed-point deposit, FP64 field). */
+void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy,
+                int nx, int ny, double lx, double ly, double dt, double qm,
+                double qp, int nsteps, double* diag);
+
+/* ---------------- collisions ---------------- */
+/* Analytic cross sections (m^2) at energy E (eV). */
+void or_mcc_cross_sections(double e_ev, double* s_el, double* s_ex, double* s_iz);
+/* Null-collision MCC on one step (mirror of the GPU kernel, FP32 particles).
+ * Returns the number of births written to the tail of the arrays (capacity cap).
+ * counts[4] receives {elastic, excitation, ionisation, null-or-none}. */
+int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, float* vy,
+                        int32_t* id, int64_t* next_id, double v_unit, double nu_max_dt_factor,
+                        double ng_sigma_scale, uint64_t seed, int64_t step, int64_t* counts);
+/* NTC hard-sphere DSMC on cell-sorted particles (mirror of the GPU kernel). */
+void or_dsmc_step_f32(int ncells, const int32_t* offsets, const int32_t* id,
+                      float* vx, float* vy, float* gmax, double coll_coeff,
+                      uint64_t seed, int64_t step, int64_t* ncoll);
+
+/* ---------------- adjoint ---------------- */
+/* Loss = sum(Ex^2+Ey^2)*dx*dy after nsteps of the FP64 collisionless run starting from
+ * x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl), v0.  Returns loss; if grad_a / grad_vx /
+ * grad_vy are non-NULL the reverse-mode gradient is written there (nmodes, n, n). */
+double or_adjoint_f64(int64_t n, const double* xl, const double* yl, const double* vx0,
+                      const double* vy0, int nmodes, const int32_t* mx, const int32_t* my,
+                      const double* amp, int nx, int ny, double lx, double ly, double dt,
+                      double qm, double qp, int nsteps,
+                      double* grad_a, double* grad_vx, double* grad_vy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

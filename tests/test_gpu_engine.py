@@ -1,0 +1,195 @@
+"""GPU parity of the full step (context API) against the CPU oracle, determinism and
+invariance properties, and the plasma physics the step must reproduce."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import pic_amd  # noqa: E402
+from pic_amd import Batch, Simulation, _lib, presets  # noqa: E402
+
+FOUR_PI = 4.0 * math.pi
+
+
+def small_landau(n=32, ppc=16, alpha=0.05, precision=_lib.PIC_FP32, **kw):
+    cfg, parts = presets.landau_small(n=n, ppc=ppc, alpha=alpha, precision=precision)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg, parts
+
+
+def test_initial_charge_and_field(oracle):
+    cfg, parts = small_landau()
+    n = parts[0].size
+    qp = -(cfg.lx * cfg.ly) / n
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        rho = sim.grid("rho")
+        ex = sim.grid("ex")
+        ey = sim.grid("ey")
+    x32, y32 = parts[0].astype(np.float32), parts[1].astype(np.float32)
+    acc = oracle.deposit_fixed(x32, y32, cfg.nx, cfg.ny, cfg.lx, cfg.ly)
+    bg = -qp * n / (cfg.lx * cfg.ly)
+    rho_r = oracle.fixed_to_rho(acc, cfg.nx, cfg.ny, cfg.lx, cfg.ly, qp) + bg
+    assert np.array_equal(rho, rho_r), "fixed-point charge must be bit-exact"
+    phi_r = oracle.poisson(rho_r, cfg.lx, cfg.ly)
+    ex_r, ey_r = oracle.efield(phi_r, cfg.lx, cfg.ly)
+    scale = max(np.abs(ex_r).max(), np.abs(ey_r).max())
+    # E is stored in FP32: half an ulp of the largest component
+    assert np.abs(ex - ex_r).max() <= 1e-6 * scale
+    assert np.abs(ey - ey_r).max() <= 1e-6 * scale
+
+
+def test_steps_match_fp32_mirror(oracle):
+    cfg, parts = small_landau()
+    n = parts[0].size
+    nsteps = 20
+    ref = [a.astype(np.float32) for a in parts]
+    diag_r = oracle.run_f32(*ref, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm,
+                            -(cfg.lx * cfg.ly) / n, nsteps)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(nsteps)
+        x, y, vx, vy = sim.particles(sync=True)
+        diag = sim.diagnostics(0, nsteps)
+    np.testing.assert_allclose(x, ref[0], atol=2e-5)
+    np.testing.assert_allclose(y, ref[1], atol=2e-5)
+    np.testing.assert_allclose(vx, ref[2], atol=2e-5)
+    np.testing.assert_allclose(vy, ref[3], atol=2e-5)
+    for col in (0, 1):  # field and kinetic energy
+        np.testing.assert_allclose(diag[:, col], diag_r[:, col], rtol=1e-5)
+    np.testing.assert_allclose(diag[:, 4], diag_r[:, 4], rtol=1e-4, atol=1e-9)
+
+
+def test_fp64_matches_oracle(oracle):
+    cfg, parts = presets.landau_small(n=32, ppc=9, alpha=0.05, precision=_lib.PIC_FP64)
+    n = parts[0].size
+    nsteps = 30
+    ref = [a.copy() for a in parts]
+    diag_r = oracle.run_f64(*ref, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm,
+                            -(cfg.lx * cfg.ly) / n, nsteps)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(nsteps)
+        x, y, vx, vy = sim.particles(sync=True)
+        diag = sim.diagnostics(0, nsteps)
+    # fixed-point weights (2^-30 of a cell) vs continuous weights
+    for a, b in zip((x, y, vx, vy), ref):
+        np.testing.assert_allclose(a, b, atol=1e-7)
+    np.testing.assert_allclose(diag[:, 0], diag_r[:, 0], rtol=1e-7)
+    np.testing.assert_allclose(diag[:, 1], diag_r[:, 1], rtol=1e-9)
+
+
+def run_state(cfg, parts, nsteps, graphs=True, tuning=None):
+    with Simulation(cfg) as sim:
+        sim.use_graphs(graphs)
+        sim.set_particles(*parts)
+        if tuning:
+            sim.set_tuning(**tuning)
+        sim.step(nsteps)
+        out = sim.particles(sync=False)
+        diag = sim.diagnostics(0, nsteps)
+    return out, diag
+
+
+def test_deterministic_and_graph_equivalent():
+    cfg, parts = small_landau(n=64, ppc=16)
+    a, da = run_state(cfg, parts, 24, graphs=True)
+    b, db = run_state(cfg, parts, 24, graphs=True)
+    c, dc = run_state(cfg, parts, 24, graphs=False)
+    for u, v, w in zip(a, b, c):
+        assert np.array_equal(u, v)
+        assert np.array_equal(u, w)
+    assert np.array_equal(da, db)
+    assert np.array_equal(da, dc)
+
+
+@pytest.mark.parametrize("tuning", [dict(tile=8, halo=4, sort_every=1),
+                                    dict(tile=32, halo=16, sort_every=3, splits=3),
+                                    dict(tile=16, halo=1, sort_every=7)])
+def test_tiling_and_sort_cadence_do_not_change_physics(tuning):
+    cfg, parts = small_landau(n=64, ppc=16)
+    ref, _ = run_state(cfg, parts, 21)
+    got, _ = run_state(cfg, parts, 21, tuning=tuning)
+    for u, v in zip(ref, got):
+        assert np.array_equal(u, v)
+
+
+def test_batch_equals_separate_runs():
+    specs = [small_landau(n=32, ppc=16, alpha=0.02), small_landau(n=64, ppc=4, alpha=0.05),
+             small_landau(n=128, ppc=2, alpha=0.01)]
+    sep = [run_state(c, p, 15)[0] for c, p in specs]
+    with Batch([c for c, _ in specs]) as b:
+        for r, (_, p) in enumerate(specs):
+            b.set_particles(*p, run=r)
+        b.step(15)
+        got = [b.particles(run=r, sync=False) for r in range(len(specs))]
+    for s, g in zip(sep, got):
+        for u, v in zip(s, g):
+            assert np.array_equal(u, v)
+
+
+def test_run_compat_matches_context():
+    cfg, parts = small_landau(n=32, ppc=16)
+    xo, yo, secs = pic_amd.run_compat(*parts, 12, cfg)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(12)
+        x, y, _, _ = sim.particles()
+    assert np.array_equal(xo, x) and np.array_equal(yo, y)
+    assert secs > 0
+
+
+def test_errors_are_reported():
+    cfg, parts = small_landau()
+    sim = Simulation(cfg)
+    with pytest.raises(pic_amd.PicError) as e:
+        sim.step(1)
+    assert e.value.code == _lib.PIC_ERR_STATE
+    sim.set_particles(*parts)
+    big = [np.zeros(cfg.capacity + 1)] * 4
+    with pytest.raises(pic_amd.PicError) as e:
+        sim.set_particles(*big)
+    assert e.value.code == _lib.PIC_ERR_OVERFLOW
+    sim.check_finite()
+    sim.close()
+
+
+def damping_rate(diag, dt, tmax=20.0):
+    amp = diag[:, 4]
+    t = np.arange(amp.size) * dt
+    pk = [i for i in range(1, amp.size - 1) if amp[i] > amp[i - 1] and amp[i] >= amp[i + 1]]
+    tp, ap = t[pk], np.log(amp[pk])
+    m = tp < tmax
+    return np.polyfit(tp[m], ap[m], 1)[0]
+
+
+@pytest.mark.slow
+def test_landau_damping_rate():
+    cfg, parts = presets.landau_small(n=128, ppc=64, alpha=0.01)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(500)
+        diag = sim.diagnostics(0, 500)
+    g = damping_rate(diag, cfg.dt)
+    assert abs(g - (-0.1534)) < 0.02, g
+    e = diag[:, 0] + diag[:, 1]
+    assert np.abs(e - e[0]).max() < 2e-3 * e[0]
+
+
+@pytest.mark.slow
+def test_two_stream_growth_rate():
+    cfg, parts, nsteps = presets.two_stream_64()
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(nsteps)
+        diag = sim.diagnostics(0, nsteps)
+    we = diag[:, 0]
+    t = np.arange(nsteps) * cfg.dt
+    lw = np.log(we)
+    # linear phase: where W_E is between 100x its floor and 1e-2 of its peak
+    m = (we > 100 * we[:20].min()) & (we < 1e-2 * we.max())
+    g = np.polyfit(t[m], lw[m], 1)[0] / 2
+    assert abs(g - 0.3406) < 0.03, g
