@@ -1,0 +1,188 @@
+"""Benchmark: particle-steps/s of the 2D electrostatic PIC step on one MI355X.
+
+Workload: collisionless Landau damping, 512x512 periodic grid, k lambda_D = 0.5, 64
+particles/cell (16,777,216), Maxwellian vth = 1, FP32 particles, FP64 FFT Poisson,
+dt = 0.05.  One "step" = field solve + fused gather/push/deposit (+ the amortised cell
+sort).  Inputs are resident in HBM when the timed region starts.
+
+Prints ONE JSON line.  Extra keys: roofline (dominant kernel, HIP-event timed), cpu_baseline
+(the CPU oracle on a bounded sample), stages (ms per step per stage), batched_vs_sequential
+(48-run ragged ensemble) and adjoint_over_forward when requested.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "msc-thesis_amd"))
+
+METRIC = "particle-steps/s at 512²×64ppc; batched vs plain; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+PMC_FILE = os.path.join(ROOT, "profiles", "push_deposit_landau512_traffic.txt")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the ensemble comparison")
+    ap.add_argument("--adjoint", action="store_true", help="also time the adjoint")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--ppc", type=int, default=64)
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per push_deposit launch measured with rocprofv3 --pmc (see the file)."""
+    try:
+        with open(PMC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(parts, cfg, nsteps):
+    """The oracle's FP32-mirror step (one thread) on the same particles, bounded sample."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import load
+    o = load()
+    n = parts[0].size
+    arrs = [a.astype(np.float32) for a in parts]
+    t0 = time.perf_counter()
+    o.run_f32(*arrs, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm, -(cfg.lx * cfg.ly) / n,
+              nsteps)
+    dt = time.perf_counter() - t0
+    return {"value": n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of the same {cfg.nx}x{cfg.ny} x {n} particle Landau "
+                      f"workload through the FP32-mirror oracle (or_run_f32, 1 thread; the "
+                      f"time includes its initial deposit and final field solve)",
+            "seconds": dt}
+
+
+def batch_ratio(steps=300):
+    """48 ragged runs: one batched context vs the same runs one at a time."""
+    import torch
+    from pic_amd import Batch, Simulation, presets
+    runs, _ = presets.batch_48()
+    total = sum(p[0].size for _, p in runs)
+    b = Batch([c for c, _ in runs])
+    for r, (_, p) in enumerate(runs):
+        b.set_particles(*p, run=r)
+    b.step(10)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    sims = []
+    for c, p in runs:
+        s = Simulation(c)
+        s.set_particles(*p)
+        s.step(10)
+        s.sync()
+        sims.append(s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(steps)
+        s.sync()
+    ts = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    return {"runs": len(runs), "particles": total, "steps": steps,
+            "batched_particle_steps_per_s": total * steps / tb,
+            "sequential_particle_steps_per_s": total * steps / ts, "ratio": ts / tb}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+              "multi-GPU sharding is a design in DESIGN.md, not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pic_amd import Simulation, presets
+
+    torch.cuda.init()
+    cfg, parts, _ = presets.landau_512(ppc=args.ppc, n=args.grid)
+    n = parts[0].size
+    sim = Simulation(cfg)
+    sim.set_particles(*parts)
+    if not args.no_autotune:
+        sim.autotune(trial_steps=8)
+    tune = sim.tuning()
+    sim.step(args.warmup)
+    sim.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = n * args.steps / el
+    ms_per_step = el * 1e3 / args.steps
+    # per-stage kernel durations, HIP events on the engine stream (eager launches)
+    stages = sim.stage_times(50)
+    sim.check_finite()
+    sim.close()
+
+    G = cfg.nx * cfg.ny
+    # push_deposit algorithmic bytes per launch: x,y,vx,vy read + written (32 B/particle),
+    # padded E read (8 B/node) and fixed-point rho written (8 B/node)
+    alg_bytes = 32.0 * n + 16.0 * G
+    push_ms = stages["push_deposit"]
+    achieved = alg_bytes / (push_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Landau quiet-start loader)",
+        "config": {"workload": f"landau_{cfg.nx}x{cfg.ny}_{args.ppc}ppc", "grid": [cfg.nx, cfg.ny],
+                   "particles": n, "dt": cfg.dt, "field": "f64 FFT Poisson",
+                   "charge": "int64 fixed-point", "tuning": tune},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_push (fused gather+push+deposit)",
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": push_ms},
+        "stages_ms": stages,
+        "algorithmic_roofline_frac": value * (40.0 + 72.0 * G / n) / (HBM_PEAK_GBS * 1e9),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(parts, cfg, 10)
+    if not args.no_batch:
+        try:
+            line["batched_vs_sequential"] = batch_ratio()
+        except Exception as e:  # reported, not hidden
+            line["batched_vs_sequential"] = {"error": repr(e)}
+    if args.adjoint:
+        try:
+            from pic_amd.adjoint import time_adjoint
+            line["adjoint_over_forward"] = time_adjoint()
+        except Exception as e:
+            line["adjoint_over_forward"] = {"error": repr(e)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

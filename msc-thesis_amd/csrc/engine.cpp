@@ -53,8 +53,8 @@ int validate_config(const pic_config& c) {
     return fail(PIC_ERR_INVALID_CONFIG, "MCC needs mcc_ng, mcc_v_unit, mcc_dt_phys, mcc_neutral_mass > 0");
   if (c.collide == PIC_COLLIDE_DSMC && (!(c.dsmc_coeff >= 0) || !(c.dsmc_gmax0 > 0)))
     return fail(PIC_ERR_INVALID_CONFIG, "DSMC needs dsmc_coeff >= 0 and dsmc_gmax0 > 0");
-  if (c.tile != 0 && (!is_pow2(c.tile) || c.tile > std::min(c.nx, c.ny) || c.tile > 64))
-    return fail(PIC_ERR_INVALID_CONFIG, "tile must be a power of two <= min(nx, ny, 64)");
+  if (c.tile != 0 && (!is_pow2(c.tile) || c.tile > std::min(c.nx, c.ny) / 2 || c.tile > 64))
+    return fail(PIC_ERR_INVALID_CONFIG, "tile must be a power of two <= min(nx, ny)/2 and <= 64");
   if (c.halo < 0 || c.splits < 0 || c.splits > 64 || c.sort_every < 0)
     return fail(PIC_ERR_INVALID_CONFIG, "halo, splits (<= 64) and sort_every must be >= 0");
   return PIC_OK;
@@ -79,11 +79,16 @@ double mcc_nu_max(const pic_config& c) {
 static Tuning default_tuning(const pic_config& c, int tile, int halo, int splits) {
   Tuning t;
   const int m = std::min(c.nx, c.ny);
-  t.tile = tile ? tile : std::min(16, m);
-  const int hmax = std::max(0, (m - t.tile) / 2);
+  t.tile = std::min(tile ? tile : 16, m / 2);
+  // at most 4096 tiles (the sort keeps one counter per tile in LDS)
+  while ((c.nx / t.tile) * (c.ny / t.tile) > 4096 && t.tile < m / 2) t.tile *= 2;
+  // the window (tile + 2 halo + 1 nodes) must not wrap onto itself: win <= min(nx, ny)
+  const int hmax = std::max(0, (m - t.tile - 1) / 2);
   t.halo = std::min(halo ? halo : 16, hmax);
   const int ntiles = (c.nx / t.tile) * (c.ny / t.tile);
-  t.splits = splits ? splits : std::max(1, std::min(8, (2048 + ntiles - 1) / ntiles));
+  // about 8192 particles per 512-thread workgroup (1 to 64 workgroups per tile)
+  const long long per_tile = std::max<long long>(c.capacity, 1) / ntiles;
+  t.splits = splits ? splits : (int)std::max(1LL, std::min(64LL, (per_tile + 8191) / 8192));
   return t;
 }
 
@@ -95,8 +100,9 @@ Ctx::~Ctx() {
   for (int i = 0; i < 2; ++i) { F(px[i]); F(py[i]); F(pvx[i]); F(pvy[i]); F(pid[i]); }
   F(rho); F(rho_lo); F(E); F(S); F(Sx); F(Sy); F(phi); F(tw);
   F(sb.keys); F(sb.rank); F(sb.perm); F(sb.counts); F(sb.offsets); F(sb.chunk_sums);
-  F(sb.big_list); F(sb.big_count); F(sb.scratch);
+  F(sb.big_list); F(sb.big_count); F(sb.scratch); F(sb.table); F(sb.tchunk_sums); F(sb.toff);
   F(d_pitems); F(d_citems); F(d_rowpair); F(d_cols); F(d_rows); F(d_push);
+  F(d_segitems); F(d_tscan_items);
   F(partials_push); F(partials_rows); F(tickets); F(diag);
   F(d_coll_counts); F(birth_count); F(birth_parent); F(birth_buf); F(gmax); F(flag);
   for (auto& e : ev) if (e) (void)hipEventDestroy(e);
@@ -118,19 +124,38 @@ int Ctx::upload_runs() {
 
 int Ctx::build_items() {
   pitems.clear(); citems.clear(); rowpair.clear(); cols.clear(); rows.clear(); push_items.clear();
+  segitems.clear(); tscan_items.clear();
   lds_push = 0;
+  long long tsum = 0, toffsum = 0;
+  int maxb = 1;
   for (int r = 0; r < nruns; ++r) {
     RunDesc& d = rd[r];
     const Tuning& t = tune[r];
     d.tile = t.tile; d.halo = t.halo; d.splits = t.splits;
     d.tiles_x = d.nx / t.tile; d.tiles_y = d.ny / t.tile;
     d.win = t.tile + 2 * t.halo + 1;
-    lds_push = std::max(lds_push, d.win * d.win * 8 * (fp64 ? 2 : 1));
+    // LDS: charge window (1 or 2 int64 words per node, padded to 16 B) + field window
+    const int nwin = d.win * d.win * (fp64 ? 2 : 1);
+    lds_push = std::max(lds_push, ((nwin + 1) & ~1) * 8 + d.win * d.win * (fp64 ? 16 : 8));
     const long long preg = round_up(std::max<long long>(d.pcap, 1), PIC_CHUNK);
+    d.nbuckets = d.tiles_x * d.tiles_y;
+    d.nchunks = (int)(preg / PIC_CHUNK);
+    if (d.nbuckets > 4096) return fail(PIC_ERR_INVALID_CONFIG, "more than 4096 tiles: use a larger tile");
+    if ((long long)d.nbuckets * d.nchunks >= (1ll << 31))
+      return fail(PIC_ERR_INVALID_CONFIG, "tile table too large: use a larger tile");
+    maxb = std::max(maxb, d.nbuckets);
+    d.tbase = tsum; tsum += (long long)d.nbuckets * d.nchunks;
+    d.toffbase = toffsum; toffsum += d.nbuckets + 1;
+    d.key_tile_major = 1;
+    d.tscan_item_begin = (int)tscan_items.size();
+    for (long long s = 0; s < (long long)d.nbuckets * d.nchunks; s += PIC_CELL_CHUNK)
+      tscan_items.push_back(make_int2(r, (int)s));
+    d.tscan_item_end = (int)tscan_items.size();
     for (long long s = 0; s < preg; s += PIC_CHUNK) pitems.push_back(make_int2(r, (int)s));
     d.cell_item_begin = (int)citems.size();
     for (int c = 0; c < d.nx * d.ny; c += PIC_CELL_CHUNK) citems.push_back(make_int2(r, c));
     d.cell_item_end = (int)citems.size();
+    for (int c = 0; c < d.nx * d.ny; c += 64) segitems.push_back(make_int2(r, c));
     d.push_item_begin = (int)push_items.size();
     for (int tx = 0; tx < d.tiles_x; ++tx)
       for (int ty = 0; ty < d.tiles_y; ++ty)
@@ -160,6 +185,20 @@ int Ctx::build_items() {
   if ((rc = up(cols, &d_cols))) return rc;
   if ((rc = up(rows, &d_rows))) return rc;
   if ((rc = up(push_items, &d_push))) return rc;
+  if ((rc = up(segitems, &d_segitems))) return rc;
+  if ((rc = up(tscan_items, &d_tscan_items))) return rc;
+  sb.segitems = d_segitems;
+  sb.nsegitems = (int)segitems.size();
+  sb.tscan_items = d_tscan_items;
+  sb.ntscan_items = (int)tscan_items.size();
+  sb.max_buckets = maxb;
+  if (sb.table) (void)hipFree(sb.table);
+  PIC_HIP(dalloc(&sb.table, tsum));
+  if (sb.tchunk_sums) (void)hipFree(sb.tchunk_sums);
+  PIC_HIP(dalloc(&sb.tchunk_sums, tscan_items.size()));
+  if (sb.toff) (void)hipFree(sb.toff);
+  PIC_HIP(dalloc(&sb.toff, toffsum));
+  PIC_HIP(hipMemset(sb.toff, 0, sizeof(int) * toffsum));
   if (partials_push) (void)hipFree(partials_push);
   PIC_HIP(dalloc(&partials_push, push_items.size() * 3));
   if (partials_rows) (void)hipFree(partials_rows);
@@ -283,7 +322,7 @@ PushArgs<R> Ctx::push_args(int mode) const {
   a.x = b.x; a.y = b.y; a.vx = b.vx; a.vy = b.vy;
   a.E = (const R*)E;
   a.rho = rho; a.rho_lo = rho_lo; a.gsum = gsum;
-  a.off = sb.offsets;
+  a.off = sb.toff;
   a.items = d_push; a.nitems = (int)push_items.size();
   a.nruns = nruns;
   a.partials = partials_push; a.ticket = tickets; a.diag = diag;
@@ -304,6 +343,18 @@ int Ctx::launch_push_stage(int mode) {
 }
 
 int Ctx::launch_sort_stage() {
+  if (!any_dsmc) {
+    // tile sort: the step only needs tiles to be contiguous
+    if (fp64)
+      launch_tile_sort<double>(stream, d_runs, d_st, nruns, bufs<double>(cur),
+                               bufs<double>(1 - cur), sb, d_pitems, (int)pitems.size());
+    else
+      launch_tile_sort<float>(stream, d_runs, d_st, nruns, bufs<float>(cur), bufs<float>(1 - cur),
+                              sb, d_pitems, (int)pitems.size());
+    cur ^= 1;
+    return PIC_OK;
+  }
+  // DSMC needs per-cell lists: full cell sort with tile-major cell numbering
   if (fp64)
     launch_sort<double>(stream, d_runs, d_st, nruns, bufs<double>(cur), bufs<double>(1 - cur), sb,
                         d_pitems, (int)pitems.size(), d_citems, (int)citems.size(), ctotal, true);
@@ -832,8 +883,8 @@ int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every)
   if (sort_every < 0 || splits < 0 || halo < 0 || tile < 0 || splits > 64)
     return fail(PIC_ERR_ARG, "tuning values must be >= 0");
   for (int r = 0; r < x.nruns; ++r) {
-    if (tile && (!is_pow2(tile) || tile > std::min(x.cfg[r].nx, x.cfg[r].ny) || tile > 64))
-      return fail(PIC_ERR_ARG, "tile must be a power of two <= min(nx, ny, 64)");
+    if (tile && (!is_pow2(tile) || tile > std::min(x.cfg[r].nx, x.cfg[r].ny) / 2 || tile > 64))
+      return fail(PIC_ERR_ARG, "tile must be a power of two <= min(nx, ny)/2 and <= 64");
   }
   PIC_HIP(hipStreamSynchronize(x.stream));
   for (int r = 0; r < x.nruns; ++r) x.tune[r] = default_tuning(x.cfg[r], tile, halo, splits);
@@ -911,15 +962,20 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
   double best = 1e30;
   int bt = 0, bh = 0, bs = 0;
   const int tiles[3] = {8, 16, 32}, halos[2] = {8, 16}, cad[4] = {1, 2, 4, 8};
+  // splits: 0 = sized by particles per tile (the only choice for ragged batches)
+  const int splitc[3] = {0, 1, 2};
   int rc = PIC_OK;
+  int bsp = 0;
   for (int ti : tiles) {
     if (ti > mn || ti > 64) continue;
     for (int h : halos) {
-      for (int se : cad) {
+      for (int se : cad)
+      for (int sp : splitc) {
         if (x.any_dsmc && se != 1) continue;
+        if (sp && x.nruns > 1) continue;
         rc = restore();
         if (rc) break;
-        for (int r = 0; r < x.nruns; ++r) x.tune[r] = default_tuning(x.cfg[r], ti, h, 0);
+        for (int r = 0; r < x.nruns; ++r) x.tune[r] = default_tuning(x.cfg[r], ti, h, sp);
         x.sort_every = se;
         if ((rc = x.build_items())) break;
         x.launch_sort_stage();
@@ -934,7 +990,7 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
         float ms = 0.f;
         PIC_HIP(hipEventElapsedTime(&ms, x.ev[0], x.ev[1]));
         const double per = ms / n;
-        if (per < best) { best = per; bt = ti; bh = h; bs = se; }
+        if (per < best) { best = per; bt = ti; bh = h; bs = se; bsp = sp; }
       }
       if (rc) break;
     }
@@ -945,7 +1001,7 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
   if (rc) { x.tune = orig; x.sort_every = orig_se; x.build_items(); return rc; }
   if (rc2) return rc2;
   if (best_ms) *best_ms = best;
-  return pic_set_tuning(ctx, bt, bh, 0, bs);
+  return pic_set_tuning(ctx, bt, bh, bsp, bs);
 }
 
 int pic_stage_times(pic_ctx* ctx, int nsteps, double* out) {

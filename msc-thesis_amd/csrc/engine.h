@@ -65,10 +65,10 @@ struct Ctx {
   double* phi = nullptr;
   double2* tw = nullptr;
   SortBufs sb{};
-  std::vector<int2> pitems, citems, rowpair, cols, rows;
+  std::vector<int2> pitems, citems, rowpair, cols, rows, segitems, tscan_items;
   std::vector<int4> push_items;
   int2 *d_pitems = nullptr, *d_citems = nullptr, *d_rowpair = nullptr, *d_cols = nullptr,
-       *d_rows = nullptr;
+       *d_rows = nullptr, *d_segitems = nullptr, *d_tscan_items = nullptr;
   int4* d_push = nullptr;
   double* partials_push = nullptr;
   double* partials_rows = nullptr;

@@ -16,6 +16,45 @@ namespace {
 // ========================================================================== sort
 #define HWIN 8192  // LDS histogram window (keys) of the rank kernel
 
+// Cell numbering of the engine's own sorts: tile-major (tile t = (cx/T)*tiles_y + cy/T,
+// then cx%T, cy%T), so that every tile and every cell is one contiguous range.  The
+// operator pic_op_sort sorts whatever keys it is given (cx*ny+cy from pic_op_cell_keys).
+__device__ __forceinline__ int engine_cell_key(int cx, int cy, const RunDesc& rd) {
+  const int T = rd.tile;
+  const int t = (cx / T) * rd.tiles_y + cy / T;
+  return t * T * T + (cx % T) * T + (cy % T);
+}
+
+__device__ __forceinline__ int tile_of(int cx, int cy, const RunDesc& rd) {
+  return (cx / rd.tile) * rd.tiles_y + cy / rd.tile;
+}
+
+// In-place exclusive scan of a[0..n) by one 256-thread workgroup (tiles of 256 values
+// with a carried prefix); returns the total in every thread.
+__device__ int block_exclusive_scan(int* a, int n, int* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += PIC_WG) {
+    const int i = base + threadIdx.x;
+    const int v = (i < n) ? a[i] : 0;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int wpre = 0;
+    for (int k = 0; k < w; ++k) wpre += wsum[k];
+    const int tile_total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (i < n) a[i] = carry + wpre + inc - v;
+    carry += tile_total;
+    __syncthreads();
+  }
+  return carry;
+}
+
 template <typename R, bool FROM_POS>
 __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict__ runs,
                                                       const RunState* __restrict__ st,
@@ -45,7 +84,7 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict_
       R f;
       locate(x[pb + p], dxr, nx, cx, f);
       locate(y[pb + p], dyr, ny, cy, f);
-      key = cx * ny + cy;
+      key = engine_cell_key(cx, cy, rd);
       keys[pb + p] = key;
     } else {
       key = keys_in[pb + p];
@@ -125,16 +164,13 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_chunks(const RunDesc* __restric
                                                         RunState* __restrict__ st,
                                                         int* __restrict__ chunk_sums,
                                                         int* __restrict__ offsets) {
+  __shared__ int wsum[4];
   const RunDesc rd = runs[blockIdx.x];
+  const int total = block_exclusive_scan(chunk_sums + rd.cell_item_begin,
+                                         rd.cell_item_end - rd.cell_item_begin, wsum);
   if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int i = rd.cell_item_begin; i < rd.cell_item_end; ++i) {
-      int v = chunk_sums[i];
-      chunk_sums[i] = acc;
-      acc += v;
-    }
-    offsets[rd.cbase + rd.nx * rd.ny] = acc;
-    st[blockIdx.x].nsorted = acc;
+    offsets[rd.cbase + rd.nx * rd.ny] = total;
+    st[blockIdx.x].nsorted = total;
   }
 }
 
@@ -176,13 +212,23 @@ __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v,
   return ((unsigned long long)hi << 32) | lo;
 }
 
+__device__ __forceinline__ unsigned long long shfl_u64(unsigned long long v, int src) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo = __shfl(lo, src, 64);
+  hi = __shfl(hi, src, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 __device__ __forceinline__ unsigned long long seg_key(const int* id, long long pb, int slot) {
   unsigned k = id ? (unsigned)id[pb + slot] : (unsigned)slot;
   return ((unsigned long long)k << 32) | (unsigned)slot;
 }
 
-// Canonical order inside each cell: ascending id.  Cells of <= 64 particles are sorted
-// by one wavefront in registers; larger cells are queued for k_segsort_big.
+// Canonical order inside each cell: ascending key (id, or the source index when id is
+// NULL, which makes the sort stable).  One wavefront per cell: <= 64 entries by a bitonic
+// network across lanes, <= 256 entries by counting ranks (4 entries per lane, keys are
+// unique); larger cells are queued for k_segsort_big.  Items cover SEG_CELLS cells.
+#define SEG_CELLS 64
 __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restrict__ runs,
                                                           const int* __restrict__ offsets,
                                                           int* __restrict__ perm,
@@ -194,18 +240,38 @@ __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restr
   const RunDesc rd = runs[it.x];
   const int G = rd.nx * rd.ny;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cend = min(it.y + PIC_CELL_CHUNK, G);
+  const int cend = min(it.y + SEG_CELLS, G);
   const long long pb = rd.pbase;
   for (int c = it.y + w; c < cend; c += 4) {
     const int b = offsets[rd.cbase + c], e = offsets[rd.cbase + c + 1];
     const int n = e - b;
     if (n <= 1) continue;
-    if (n > 64) {
+    if (n > 256) {
       if (lane == 0) {
         int slot = atomicAdd(big_count, 1);
         big_list[2 * slot] = it.x;
         big_list[2 * slot + 1] = c;
       }
+      continue;
+    }
+    if (n > 64) {
+      unsigned long long k4[4];
+      int r4[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int j = lane + 64 * k;
+        k4[k] = (j < n) ? seg_key(id, pb, perm[pb + b + j]) : ~0ull;
+      }
+      for (int jj = 0; jj < n; ++jj) {
+        const int src = jj >> 6;
+        const unsigned long long sel = src == 0 ? k4[0] : (src == 1 ? k4[1] : (src == 2 ? k4[2] : k4[3]));
+        const unsigned long long kj = shfl_u64(sel, jj & 63);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r4[k] += (kj < k4[k]) ? 1 : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (lane + 64 * k < n) perm[pb + b + r4[k]] = (int)(unsigned)(k4[k] & 0xffffffffull);
       continue;
     }
     unsigned long long v = (lane < n) ? seg_key(id, pb, perm[pb + b + lane]) : ~0ull;
@@ -285,6 +351,172 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict
   }
 }
 
+// ------------------------------------------------------------ tile sort of the step
+// Stable counting sort of the particles by tile (sort bucket):
+//   1. k_tile_hist    per PIC_CHUNK chunk: count of each bucket -> table[b][chunk]
+//   2. k_tscan_*      exclusive scan of the table in bucket-major order (per run)
+//   3. k_tile_scatter one wavefront per chunk walks its particles in index order; lanes
+//                     with equal buckets get consecutive slots (ballot ranks), so the
+//                     order inside a bucket is the previous order: deterministic and
+//                     stable without any per-cell sort.
+template <typename R>
+__device__ __forceinline__ int bucket_of(R x, R y, const RunDesc& rd) {
+  int cx, cy;
+  R f;
+  locate(x, (R)rd.dx, rd.nx, cx, f);
+  locate(y, (R)rd.dy, rd.ny, cy, f);
+  return tile_of(cx, cy, rd);
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict__ runs,
+                                                      const RunState* __restrict__ st,
+                                                      const R* __restrict__ x,
+                                                      const R* __restrict__ y,
+                                                      int* __restrict__ table,
+                                                      const int2* __restrict__ items) {
+  extern __shared__ int h[];
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int np = st[it.x].np;
+  const int nb = rd.nbuckets;
+  const int chunk = it.y / PIC_CHUNK;
+  for (int b = threadIdx.x; b < nb; b += PIC_WG) h[b] = 0;
+  __syncthreads();
+  const int p1 = min(it.y + PIC_CHUNK, np);
+  for (int p = it.y + threadIdx.x; p < p1; p += PIC_WG)
+    atomicAdd(&h[bucket_of<R>(x[rd.pbase + p], y[rd.pbase + p], rd)], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < nb; b += PIC_WG)
+    table[rd.tbase + (long long)b * rd.nchunks + chunk] = h[b];
+}
+
+// exclusive scan of one PIC_CELL_CHUNK slice of a run's table, in place
+__global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restrict__ runs,
+                                                        int* __restrict__ table,
+                                                        int* __restrict__ chunk_sums,
+                                                        const int2* __restrict__ items) {
+  __shared__ int wsum[4];
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const long long len = (long long)rd.nbuckets * rd.nchunks;
+  const long long c0 = it.y + threadIdx.x * 8;
+  int v[8], s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = (c0 + k < len) ? table[rd.tbase + c0 + k] : 0;
+    s += v[k];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int wpre = 0;
+  for (int k = 0; k < w; ++k) wpre += wsum[k];
+  int ex = wpre + inc - s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (c0 + k < len) table[rd.tbase + c0 + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_tscan_chunks(const RunDesc* __restrict__ runs,
+                                                         int* __restrict__ chunk_sums) {
+  __shared__ int wsum[4];
+  const RunDesc rd = runs[blockIdx.x];
+  block_exclusive_scan(chunk_sums + rd.tscan_item_begin,
+                       rd.tscan_item_end - rd.tscan_item_begin, wsum);
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict__ runs,
+                                                      int* __restrict__ table,
+                                                      const int* __restrict__ chunk_sums,
+                                                      const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const long long len = (long long)rd.nbuckets * rd.nchunks;
+  const int add = chunk_sums[blockIdx.x];
+  for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
+    const long long c = (long long)it.y + k;
+    if (c < len) table[rd.tbase + c] += add;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__ runs,
+                                                     RunState* __restrict__ st,
+                                                     PartBufs<R> src, PartBufs<R> dst,
+                                                     const int* __restrict__ table,
+                                                     int* __restrict__ toff,
+                                                     const int2* __restrict__ items) {
+  extern __shared__ int base[];
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int np = st[it.x].np;
+  const int nb = rd.nbuckets;
+  const int chunk = it.y / PIC_CHUNK;
+  const int lane = threadIdx.x;
+  for (int b = lane; b < nb; b += 64) {
+    const int v = table[rd.tbase + (long long)b * rd.nchunks + chunk];
+    base[b] = v;
+    if (chunk == 0) toff[rd.toffbase + b] = v;
+  }
+  if (chunk == 0 && lane == 0) {
+    toff[rd.toffbase + nb] = np;
+    st[it.x].nsorted = np;
+  }
+  __syncthreads();
+  const long long pb = rd.pbase;
+  const int p1 = min(it.y + PIC_CHUNK, np);
+  for (int q = it.y; q < p1; q += 64) {
+    const int p = q + lane;
+    const bool act = p < p1;
+    R vx = 0, vy = 0, vvx = 0, vvy = 0;
+    int vid = 0, key = -1;
+    if (act) {
+      vx = src.x[pb + p]; vy = src.y[pb + p]; vvx = src.vx[pb + p]; vvy = src.vy[pb + p];
+      vid = src.id[pb + p];
+      key = bucket_of<R>(vx, vy, rd);
+    }
+    unsigned long long rem = __ballot(act);
+    int d = 0;
+    while (rem) {
+      const int leader = __ffsll((long long)rem) - 1;
+      const int lk = __shfl(key, leader, 64);
+      const unsigned long long m = __ballot(act && key == lk);
+      const int b0 = base[lk];
+      if (act && key == lk) d = b0 + __popcll(m & ((1ull << lane) - 1ull));
+      __syncthreads();
+      if (lane == leader) base[lk] = b0 + __popcll(m);
+      __syncthreads();
+      rem &= ~m;
+    }
+    if (act) {
+      dst.x[pb + d] = vx; dst.y[pb + d] = vy; dst.vx[pb + d] = vvx; dst.vy[pb + d] = vvy;
+      dst.id[pb + d] = vid;
+    }
+  }
+}
+
+// tile offsets of a tile-major cell sort: toff[t] = offsets[t * T * T]
+__global__ __launch_bounds__(PIC_WG) void k_toff_from_cells(const RunDesc* __restrict__ runs,
+                                                            const int* __restrict__ offsets,
+                                                            int* __restrict__ toff, int nruns) {
+  for (int r = 0; r < nruns; ++r) {
+    const RunDesc rd = runs[r];
+    const int TT = rd.tile * rd.tile;
+    for (int t = blockIdx.x * PIC_WG + threadIdx.x; t <= rd.nbuckets; t += gridDim.x * PIC_WG)
+      toff[rd.toffbase + t] = offsets[rd.cbase + (long long)t * TT];
+  }
+}
+
 __global__ void k_fill_i32(int* p, long long n, int v) {
   for (long long i = blockIdx.x * (long long)PIC_WG + threadIdx.x; i < n;
        i += (long long)gridDim.x * PIC_WG)
@@ -342,10 +574,18 @@ __device__ __forceinline__ void deposit_global(unsigned long long* rho, unsigned
 }
 
 // gather + kick + drift of one particle; returns synchronous-velocity sums
+// E window in LDS: ew[r*W + c] = E at node (x0 + r, y0 + c) (periodic), or NULL
+template <typename R>
+struct EWin {
+  typedef typename std::conditional<sizeof(R) == 4, float2, double2>::type R2;
+  const R2* ew;
+  int W, x0, y0;
+};
+
 template <typename R>
 __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* __restrict__ E,
                                          const Geo<R>& g, R hk, R dtr, double skc, double& ke,
-                                         double& px, double& py) {
+                                         double& px, double& py, const EWin<R>& w) {
   int cx, cy;
   R fx, fy;
   locate(xr, g.dx, g.nx, cx, fx);
@@ -354,10 +594,19 @@ __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* 
   const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
   const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
   const R g00 = gx0 * gy0, g10 = gx1 * gy0, g01 = gx0 * gy1, g11 = gx1 * gy1;
-  typedef typename std::conditional<sizeof(R) == 4, float2, double2>::type R2;
-  const R2* e = reinterpret_cast<const R2*>(E);
-  const long long i00 = (long long)cx * (g.ny + 1) + cy;
-  const R2 e00 = e[i00], e01 = e[i00 + 1], e10 = e[i00 + g.ny + 1], e11 = e[i00 + g.ny + 2];
+  typedef typename EWin<R>::R2 R2;
+  R2 e00, e01, e10, e11;
+  int rx = cx - w.x0, ry = cy - w.y0;
+  rx += (rx < 0) ? g.nx : ((rx >= g.nx) ? -g.nx : 0);
+  ry += (ry < 0) ? g.ny : ((ry >= g.ny) ? -g.ny : 0);
+  if (w.ew && rx < w.W - 1 && ry < w.W - 1) {
+    const int i0 = rx * w.W + ry;
+    e00 = w.ew[i0]; e01 = w.ew[i0 + 1]; e10 = w.ew[i0 + w.W]; e11 = w.ew[i0 + w.W + 1];
+  } else {
+    const R2* e = reinterpret_cast<const R2*>(E);
+    const long long i00 = (long long)cx * (g.ny + 1) + cy;
+    e00 = e[i00]; e01 = e[i00 + 1]; e10 = e[i00 + g.ny + 1]; e11 = e[i00 + g.ny + 2];
+  }
   R epx = g00 * e00.x;
   epx = epx + g10 * e10.x;
   epx = epx + g01 * e01.x;
@@ -376,165 +625,7 @@ __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* 
   yr = wrap_pos((R)(yr + dtr * vyr), g.Ly);
 }
 
-template <typename R, int MODE>
-__global__ __launch_bounds__(PIC_WG) void k_push(PushArgs<R> a) {
-  extern __shared__ unsigned long long win[];
-  __shared__ int colstart[65];
-  __shared__ int colbase[64];
-  __shared__ double red[4];
-  __shared__ unsigned int ticket_s;
-  const int4 it = a.items[blockIdx.x];
-  const int run = it.x;
-  const RunDesc rd = a.runs[run];
-  const RunState rs = a.st[run];
-  Geo<R> g;
-  g.nx = rd.nx; g.ny = rd.ny;
-  g.dx = (R)rd.dx; g.dy = (R)rd.dy; g.Lx = (R)rd.lx; g.Ly = (R)rd.ly;
-  const long long pb = rd.pbase;
-  R* __restrict__ X = a.x + pb;
-  R* __restrict__ Y = a.y + pb;
-  R* __restrict__ VX = a.vx + pb;
-  R* __restrict__ VY = a.vy + pb;
-  const R* __restrict__ E = a.E + 2 * rd.ebase;
-  const int par = (MODE == 0) ? (int)((rs.step + 1) & 1) : (int)(rs.step & 1);
-  unsigned long long* rho = a.rho + par * a.gsum + rd.gbase;
-  unsigned long long* rlo = (sizeof(R) == 8) ? a.rho_lo + par * a.gsum + rd.gbase : nullptr;
-  const double c = rs.vsync ? 0.5 : 1.0;
-  const R hk = (R)(c * rd.dt * rd.qm), dtr = (R)rd.dt;
-  const double skc = (c - 0.5) * rd.dt * rd.qm;
-  double ke = 0.0, px = 0.0, py = 0.0;
-
-  if (it.y >= 0) {
-    const int T = rd.tile, H = rd.halo, W = rd.win, S = rd.splits;
-    const int nwin = W * W * ((sizeof(R) == 8) ? 2 : 1);
-    for (int i = threadIdx.x; i < nwin; i += PIC_WG) win[i] = 0ull;
-    const int tx = it.y, ty = it.z, split = it.w;
-    if (threadIdx.x < T) {
-      const long long cb = rd.cbase + (long long)(tx * T + threadIdx.x) * g.ny + ty * T;
-      const int s0 = a.off[cb], s1 = a.off[cb + T];
-      colbase[threadIdx.x] = s0;
-      colstart[threadIdx.x + 1] = s1 - s0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      colstart[0] = 0;
-      for (int i = 0; i < T; ++i) colstart[i + 1] += colstart[i];
-    }
-    __syncthreads();
-    const long long F = colstart[T];
-    const int f0 = (int)(F * split / S), f1 = (int)(F * (split + 1) / S);
-    const int x0 = tx * T - H, y0 = ty * T - H;
-    unsigned long long* wlo = win + W * W;
-    for (int f = f0 + threadIdx.x; f < f1; f += PIC_WG) {
-      int lo = 0, hi = T - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (colstart[mid] <= f) lo = mid; else hi = mid - 1;
-      }
-      const int p = colbase[lo] + (f - colstart[lo]);
-      R xr = X[p], yr = Y[p];
-      if (MODE == 0) {
-        R vxr = VX[p], vyr = VY[p];
-        push_one<R>(xr, yr, vxr, vyr, E, g, hk, dtr, skc, ke, px, py);
-        X[p] = xr; Y[p] = yr; VX[p] = vxr; VY[p] = vyr;
-      }
-      int cx, cy;
-      long long w[4];
-      deposit_weights<R>(xr, yr, g, cx, cy, w);
-      int rx = cx - x0, ry = cy - y0;
-      rx += (rx < 0) ? g.nx : ((rx >= g.nx) ? -g.nx : 0);
-      ry += (ry < 0) ? g.ny : ((ry >= g.ny) ? -g.ny : 0);
-      if (rx < W - 1 && ry < W - 1) {
-        const int i0 = rx * W + ry;
-        if (sizeof(R) == 8) {
-          const long long m = (1ll << 30) - 1;
-          atomicAdd(&win[i0], (unsigned long long)(w[0] >> 30));
-          atomicAdd(&win[i0 + W], (unsigned long long)(w[1] >> 30));
-          atomicAdd(&win[i0 + 1], (unsigned long long)(w[2] >> 30));
-          atomicAdd(&win[i0 + W + 1], (unsigned long long)(w[3] >> 30));
-          atomicAdd(&wlo[i0], (unsigned long long)(w[0] & m));
-          atomicAdd(&wlo[i0 + W], (unsigned long long)(w[1] & m));
-          atomicAdd(&wlo[i0 + 1], (unsigned long long)(w[2] & m));
-          atomicAdd(&wlo[i0 + W + 1], (unsigned long long)(w[3] & m));
-        } else {
-          atomicAdd(&win[i0], (unsigned long long)w[0]);
-          atomicAdd(&win[i0 + W], (unsigned long long)w[1]);
-          atomicAdd(&win[i0 + 1], (unsigned long long)w[2]);
-          atomicAdd(&win[i0 + W + 1], (unsigned long long)w[3]);
-        }
-      } else {
-        deposit_global<R>(rho, rlo, cx, cy, w, g);
-      }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < W * W; i += PIC_WG) {
-      const unsigned long long v = win[i];
-      const unsigned long long vl = (sizeof(R) == 8) ? wlo[i] : 0ull;
-      if (v | vl) {
-        int gx = (x0 + i / W) % g.nx, gy = (y0 + i % W) % g.ny;
-        gx += (gx < 0) ? g.nx : 0;
-        gy += (gy < 0) ? g.ny : 0;
-        const long long nd = (long long)gx * g.ny + gy;
-        if (v) atomicAdd(&rho[nd], v);
-        if (sizeof(R) == 8 && vl) atomicAdd(&rlo[nd], vl);
-      }
-    }
-  } else {
-    // tail chunk: particles appended after the last sort (not covered by the offsets)
-    const int p0 = max(it.z, rs.nsorted), p1 = min(it.z + PIC_CHUNK, rs.np);
-    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) {
-      R xr = X[p], yr = Y[p];
-      if (MODE == 0) {
-        R vxr = VX[p], vyr = VY[p];
-        push_one<R>(xr, yr, vxr, vyr, E, g, hk, dtr, skc, ke, px, py);
-        X[p] = xr; Y[p] = yr; VX[p] = vxr; VY[p] = vyr;
-      }
-      int cx, cy;
-      long long w[4];
-      deposit_weights<R>(xr, yr, g, cx, cy, w);
-      deposit_global<R>(rho, rlo, cx, cy, w, g);
-    }
-  }
-  if (MODE != 0) return;
-
-  // per-workgroup partial sums, then the last workgroup reduces them in item order
-  const double ske = block_sum(ke, red);
-  const double spx = block_sum(px, red);
-  const double spy = block_sum(py, red);
-  if (threadIdx.x == 0) {
-    a.partials[3 * (long long)blockIdx.x + 0] = ske;
-    a.partials[3 * (long long)blockIdx.x + 1] = spx;
-    a.partials[3 * (long long)blockIdx.x + 2] = spy;
-    __threadfence();
-    ticket_s = atomicAdd(a.ticket, 1u);
-  }
-  __syncthreads();
-  if (ticket_s != gridDim.x - 1) return;
-  __threadfence();
-  for (int r = 0; r < a.nruns; ++r) {
-    const RunDesc d = a.runs[r];
-    double s3[3];
-    for (int k = 0; k < 3; ++k) {
-      double v = 0.0;
-      for (int i = d.push_item_begin + threadIdx.x; i < d.push_item_end; i += PIC_WG)
-        v += a.partials[3 * (long long)i + k];
-      s3[k] = block_sum(v, red);
-    }
-    if (threadIdx.x == 0) {
-      RunState* sr = a.st + r;
-      const long long stp = sr->step;
-      const double mp = d.qp / d.qm;
-      double* row = a.diag + (d.dbase + (stp % PIC_DIAG_CAP)) * 8;
-      row[1] = 0.5 * mp * s3[0];
-      row[2] = mp * s3[1];
-      row[3] = mp * s3[2];
-      row[5] = (double)sr->np;
-      sr->step = stp + 1;
-      sr->vsync = 0;
-    }
-  }
-  if (threadIdx.x == 0) *a.ticket = 0u;
-}
+#include "push_kernel.inc"
 
 // synchronous velocities for output: v + 1/2 dt qm E(x) when a half kick is pending
 template <typename R>
@@ -560,7 +651,9 @@ __global__ __launch_bounds__(PIC_WG) void k_half_kick_out(const RunDesc* __restr
     R xr = p.x[i], yr = p.y[i], vxr = p.vx[i], vyr = p.vy[i];
     double ke = 0, px = 0, py = 0;
     R x2 = xr, y2 = yr;
-    push_one<R>(x2, y2, vxr, vyr, E, g, hk, (R)0, 0.0, ke, px, py);
+    EWin<R> none;
+    none.ew = nullptr; none.W = 0; none.x0 = 0; none.y0 = 0;
+    push_one<R>(x2, y2, vxr, vyr, E, g, hk, (R)0, 0.0, ke, px, py, none);
     ovx[q] = vxr;
     ovy[q] = vyr;
   }
@@ -762,8 +855,8 @@ void launch_reduce_sum(hipStream_t s, const double* parts, int n, double* out) {
 static void sort_tail(hipStream_t s, const RunDesc* runs, RunState* st, const int* id,
                       SortBufs sb, const int2* citems, int ncitems) {
   launch_fill_i32(s, sb.big_count, 1, 0);
-  k_segsort_small<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list,
-                                             sb.big_count, citems);
+  k_segsort_small<<<sb.nsegitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list,
+                                                  sb.big_count, sb.segitems);
   k_segsort_big<<<256, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list, sb.big_count,
                                        sb.scratch);
 }
@@ -788,6 +881,20 @@ void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, Pa
   sort_tail(s, runs, st, src.id, sb, citems, ncitems);
   if (move_data)
     k_sort_apply<R><<<npitems, PIC_WG, 0, s>>>(runs, st, sb.perm, src, dst, pitems);
+  k_toff_from_cells<<<64, PIC_WG, 0, s>>>(runs, sb.offsets, sb.toff, nruns);
+}
+
+template <typename R>
+void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                      PartBufs<R> src, PartBufs<R> dst, SortBufs sb, const int2* pitems,
+                      int npitems) {
+  const size_t lds = sizeof(int) * (size_t)sb.max_buckets;
+  k_tile_hist<R><<<npitems, PIC_WG, lds, s>>>(runs, st, src.x, src.y, sb.table, pitems);
+  k_tscan_local<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
+                                                   sb.tscan_items);
+  k_tscan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
+  k_tscan_add<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums, sb.tscan_items);
+  k_tile_scatter<R><<<npitems, 64, lds, s>>>(runs, st, src, dst, sb.table, sb.toff, pitems);
 }
 
 void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,
@@ -805,9 +912,9 @@ template <typename R>
 void launch_push(hipStream_t s, const PushArgs<R>& a) {
   if (a.nitems <= 0) return;
   if (a.mode == 0)
-    k_push<R, 0><<<a.nitems, PIC_WG, a.lds_bytes, s>>>(a);
+    k_push<R, 0><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
   else
-    k_push<R, 1><<<a.nitems, PIC_WG, a.lds_bytes, s>>>(a);
+    k_push<R, 1><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
 }
 
 template <typename R>
@@ -848,6 +955,8 @@ void launch_op_gather_push(hipStream_t s, R* x, R* y, R* vx, R* vy, int64_t n, c
   template void launch_sort<R>(hipStream_t, const RunDesc*, RunState*, int, PartBufs<R>,    \
                                PartBufs<R>, SortBufs, const int2*, int, const int2*, int,   \
                                long long, bool);                                            \
+  template void launch_tile_sort<R>(hipStream_t, const RunDesc*, RunState*, int, PartBufs<R>,\
+                                    PartBufs<R>, SortBufs, const int2*, int);                 \
   template void launch_push<R>(hipStream_t, const PushArgs<R>&);                            \
   template void launch_half_kick_out<R>(hipStream_t, const RunDesc*, RunState*, int,        \
                                         PartBufs<R>, const R*, R*, R*, const int2*, int);   \

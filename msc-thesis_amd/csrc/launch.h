@@ -20,6 +20,15 @@ struct SortBufs {
   int* big_list;    // [ptotal] cells needing the workgroup segment sort (run, cell) pairs
   int* big_count;   // [1]
   unsigned long long* scratch;  // [2*ptotal] power-of-two padded segments
+  const int2* segitems;         // {run, first cell} items of 64 cells (segment sort)
+  int nsegitems;
+  // tile sort of the step
+  int* table;                   // [sum nbuckets*nchunks] bucket-major count table
+  int* tchunk_sums;             // [n table-scan items]
+  const int2* tscan_items;      // {run, first entry} items of PIC_CELL_CHUNK table entries
+  int ntscan_items;
+  int* toff;                    // [sum (nbuckets+1)] tile offsets
+  int max_buckets;
 };
 
 template <typename R>
@@ -34,6 +43,11 @@ void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
                  PartBufs<R> src, PartBufs<R> dst, SortBufs sb,
                  const int2* pitems, int npitems, const int2* citems, int ncitems,
                  long long ctotal, bool move_data);
+// stable tile (bucket) sort of the step; writes sb.toff and RunState::nsorted
+template <typename R>
+void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                      PartBufs<R> src, PartBufs<R> dst, SortBufs sb, const int2* pitems,
+                      int npitems);
 // op: keys given (run 0 layout), produce perm/counts/offsets
 void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,
                            SortBufs sb, const int2* pitems, int npitems, const int2* citems,

@@ -98,16 +98,21 @@ int pic_op_sort(const int32_t* keys, int64_t n, int ncells, int32_t* perm, int32
   RunState hs;
   std::memset(&hs, 0, sizeof(hs));
   hs.np = (int)n;
-  std::vector<int2> pit, cit;
+  std::vector<int2> pit, cit, sit;
   for (long long q = 0; q < std::max<long long>(n, 1); q += PIC_CHUNK) pit.push_back(make_int2(0, (int)q));
   for (int c = 0; c < ncells; c += PIC_CELL_CHUNK) cit.push_back(make_int2(0, c));
+  for (int c = 0; c < ncells; c += 64) sit.push_back(make_int2(0, c));
   d.cell_item_begin = 0;
   d.cell_item_end = (int)cit.size();
-  int2 *dp, *dc;
+  int2 *dp, *dc, *ds;
   PIC_HIP(m.get(&dp, pit.size()));
   PIC_HIP(m.get(&dc, cit.size()));
+  PIC_HIP(m.get(&ds, sit.size()));
+  PIC_HIP(hipMemcpyAsync(ds, sit.data(), sizeof(int2) * sit.size(), hipMemcpyHostToDevice, s));
   SortBufs sb;
   std::memset(&sb, 0, sizeof(sb));
+  sb.segitems = ds;
+  sb.nsegitems = (int)sit.size();
   PIC_HIP(m.get(&sb.rank, std::max<int64_t>(n, 1)));
   PIC_HIP(m.get(&sb.chunk_sums, cit.size()));
   PIC_HIP(m.get(&sb.big_list, 2 * (size_t)ncells));

@@ -27,6 +27,11 @@ struct RunDesc {
   int row_item_begin, row_item_end;  // field-kernel row items of this run
   int cell_item_begin, cell_item_end;
   int pcap, pad0;
+  int nbuckets, nchunks;             // tiles (sort buckets) and PIC_CHUNK particle chunks
+  int tscan_item_begin, tscan_item_end;
+  int key_tile_major, pad1;          // 1: cell keys ordered tile-major (engine sorts)
+  long long tbase;                   // bucket x chunk count table base (nbuckets*nchunks)
+  long long toffbase;                // tile offsets base (nbuckets + 1)
   double lx, ly, dx, dy;
   double dt, qm, qp;
   double rho_scale;                  // charge per area of one fixed-point unit

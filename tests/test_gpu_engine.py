@@ -168,28 +168,29 @@ def damping_rate(diag, dt, tmax=20.0):
 
 @pytest.mark.slow
 def test_landau_damping_rate():
-    cfg, parts = presets.landau_small(n=128, ppc=64, alpha=0.01)
+    # k lambda_D = 0.5: omega = 1.4157 - 0.1534i (Faddeeva root).  The |rho_hat(k)| peaks
+    # are fitted while the mode stays well above the particle noise floor (t < 12).
+    cfg, parts = presets.landau_small(n=256, ppc=64, alpha=0.05)
     with Simulation(cfg) as sim:
         sim.set_particles(*parts)
-        sim.step(500)
-        diag = sim.diagnostics(0, 500)
-    g = damping_rate(diag, cfg.dt)
-    assert abs(g - (-0.1534)) < 0.02, g
+        sim.step(300)
+        diag = sim.diagnostics(0, 300)
+    g = damping_rate(diag, cfg.dt, tmax=12.0)
+    assert abs(g - (-0.1534)) < 0.025, g
     e = diag[:, 0] + diag[:, 1]
     assert np.abs(e - e[0]).max() < 2e-3 * e[0]
 
 
 @pytest.mark.slow
 def test_two_stream_growth_rate():
+    # cold symmetric beams, k v0 = 0.5: gamma = 0.34063; fitted on ln|rho_hat(k)| over the
+    # linear phase t in [8, 16] of the 64x64, 100 particles/cell, dt = 0.1 workload
     cfg, parts, nsteps = presets.two_stream_64()
     with Simulation(cfg) as sim:
         sim.set_particles(*parts)
         sim.step(nsteps)
         diag = sim.diagnostics(0, nsteps)
-    we = diag[:, 0]
     t = np.arange(nsteps) * cfg.dt
-    lw = np.log(we)
-    # linear phase: where W_E is between 100x its floor and 1e-2 of its peak
-    m = (we > 100 * we[:20].min()) & (we < 1e-2 * we.max())
-    g = np.polyfit(t[m], lw[m], 1)[0] / 2
+    m = (t >= 8) & (t <= 16)
+    g = np.polyfit(t[m], np.log(diag[m, 4]), 1)[0]
     assert abs(g - 0.3406) < 0.03, g
