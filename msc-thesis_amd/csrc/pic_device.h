@@ -124,14 +124,16 @@ __device__ __forceinline__ void sincos2pi_own(double u, double& s, double& c) {
   int oct = (int)q & 7;
   double a = ((oct & 1) ? (1.0 - r) : r) * (kPi / 4.0);
   double a2 = a * a;
-  double ps = 1.0 / 6227020800.0;
+  double ps = 1.0 / 1307674368000.0;
+  ps = fma(ps, -a2, 1.0 / 6227020800.0);
   ps = fma(ps, -a2, 1.0 / 39916800.0);
   ps = fma(ps, -a2, 1.0 / 362880.0);
   ps = fma(ps, -a2, 1.0 / 5040.0);
   ps = fma(ps, -a2, 1.0 / 120.0);
   ps = fma(ps, -a2, 1.0 / 6.0);
   double sa = fma(-a * a2, ps, a);
-  double pc = 1.0 / 87178291200.0;
+  double pc = 1.0 / 20922789888000.0;
+  pc = fma(pc, -a2, 1.0 / 87178291200.0);
   pc = fma(pc, -a2, 1.0 / 479001600.0);
   pc = fma(pc, -a2, 1.0 / 3628800.0);
   pc = fma(pc, -a2, 1.0 / 40320.0);

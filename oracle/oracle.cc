@@ -77,15 +77,17 @@ void sincos2pi(double u, double* s, double* c) {
   // angle inside octant, measured so that the polynomial argument is in [0, pi/4]
   double a = ((oct & 1) ? (1.0 - r) : r) * (kPi / 4.0);
   double a2 = a * a;
-  // sin(a) = a - a^3/3! + ... ; cos(a) = 1 - a^2/2! + ...   (|a| <= pi/4, 11 terms)
-  double ps = 1.0 / 6227020800.0;  // 1/13!
+  // sin(a) = a - a^3/3! + ... ; cos(a) = 1 - a^2/2! + ...   (|a| <= pi/4, to a^15, a^16)
+  double ps = 1.0 / 1307674368000.0;  // 1/15!
+  ps = std::fma(ps, -a2, 1.0 / 6227020800.0);
   ps = std::fma(ps, -a2, 1.0 / 39916800.0);
   ps = std::fma(ps, -a2, 1.0 / 362880.0);
   ps = std::fma(ps, -a2, 1.0 / 5040.0);
   ps = std::fma(ps, -a2, 1.0 / 120.0);
   ps = std::fma(ps, -a2, 1.0 / 6.0);
   double sa = std::fma(-a * a2, ps, a);
-  double pc = 1.0 / 87178291200.0;  // 1/14!
+  double pc = 1.0 / 20922789888000.0;  // 1/16!
+  pc = std::fma(pc, -a2, 1.0 / 87178291200.0);
   pc = std::fma(pc, -a2, 1.0 / 479001600.0);
   pc = std::fma(pc, -a2, 1.0 / 3628800.0);
   pc = std::fma(pc, -a2, 1.0 / 40320.0);
@@ -707,5 +709,118 @@ void or_mcc_cross_sections(double e, double* s_el, double* s_ex, double* s_iz) {
   *s_ex = (e > 11.5) ? 2e-21 * (1.0 - 11.5 / e) : 0.0;
   *s_iz = (e > 15.8) ? 3e-20 * log_own(e / 15.8) / e : 0.0;
 }
+
+// MCC slot of thesisCode/mccCollisions/src/particle.cc:112-127 (collision after the push),
+// with null-collision physics: process chosen by u * nu_max against cumulative nu_k(E).
+int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, float* vy,
+                        int32_t* id, int64_t next_id, double e_per_v2,
+                        double nu_dt_per_sigma_v, double p_coll, double nu_max_dt,
+                        double mass_ratio2, uint64_t seed, int64_t step, int64_t* counts) {
+  struct Birth { int32_t parent; float x, y, vx, vy; };
+  std::vector<Birth> births;
+  const uint32_t s32 = (uint32_t)step;
+  for (int64_t p = 0; p < n; ++p) {
+    double u[4];
+    draw4((uint32_t)id[p], s32, 0u, TAG_MCC, seed, u);
+    if (u[0] >= p_coll) continue;
+    double vxd = (double)vx[p], vyd = (double)vy[p];
+    const double v2 = vxd * vxd + vyd * vyd;
+    const double sp = std::sqrt(v2);
+    const double e = e_per_v2 * v2;
+    double sel, sex, siz;
+    or_mcc_cross_sections(e, &sel, &sex, &siz);
+    const double nel = nu_dt_per_sigma_v * sel * sp;
+    const double nex = nu_dt_per_sigma_v * sex * sp;
+    const double niz = nu_dt_per_sigma_v * siz * sp;
+    const double r = u[1] * nu_max_dt;
+    double sn, cs;
+    sincos2pi(u[2], &sn, &cs);
+    if (r < nel) {
+      const double f = std::sqrt(1.0 - mass_ratio2 * (1.0 - cs));
+      const double nvx = f * (vxd * cs - vyd * sn), nvy = f * (vxd * sn + vyd * cs);
+      vxd = nvx; vyd = nvy;
+      counts[0]++;
+    } else if (r < nel + nex) {
+      const double f = std::sqrt((e - 11.5) / e);
+      const double nvx = f * (vxd * cs - vyd * sn), nvy = f * (vxd * sn + vyd * cs);
+      vxd = nvx; vyd = nvy;
+      counts[1]++;
+    } else if (r < nel + nex + niz) {
+      const double ea = e - 15.8;
+      const double e1 = u[3] * ea, e2 = ea - e1;
+      const double f1 = std::sqrt(e1 / e);
+      const double nvx = f1 * (vxd * cs - vyd * sn), nvy = f1 * (vxd * sn + vyd * cs);
+      double w[4];
+      draw4((uint32_t)id[p], s32, 1u, TAG_MCC, seed, w);
+      double sn2, cs2;
+      sincos2pi(w[0], &sn2, &cs2);
+      const double sp2 = sp * std::sqrt(e2 / e);
+      births.push_back({id[p], x[p], y[p], (float)(sp2 * cs2), (float)(sp2 * sn2)});
+      vxd = nvx; vyd = nvy;
+      counts[2]++;
+    } else {
+      counts[3]++;
+      continue;
+    }
+    vx[p] = (float)vxd;
+    vy[p] = (float)vyd;
+  }
+  std::sort(births.begin(), births.end(),
+            [](const Birth& a, const Birth& b) { return a.parent < b.parent; });
+  int64_t m = n;
+  for (size_t k = 0; k < births.size() && m < cap; ++k, ++m) {
+    x[m] = births[k].x; y[m] = births[k].y; vx[m] = births[k].vx; vy[m] = births[k].vy;
+    id[m] = (int32_t)(next_id + (int64_t)k);
+  }
+  return m;
+}
+
+// DSMC slot of thesisCode/dsmcCollisions/cellTrackerAdvanced/src/main.cc:288-350, with NTC
+// hard-sphere pair selection (Bird): M = 1/2 N (N-1) coeff gmax candidates per cell, each
+// accepted with probability g/gmax; post-collision relative velocity rotated isotropically.
+int64_t or_dsmc_step_f32(int ncells, const int32_t* offsets, float* vx, float* vy,
+                         float* gmax, double coeff, uint64_t seed, int64_t step) {
+  const uint32_t s32 = (uint32_t)step;
+  int64_t total = 0;
+  for (int c = 0; c < ncells; ++c) {
+    const int b = offsets[c], n = offsets[c + 1] - b;
+    if (n < 2) continue;
+    double gm = (double)gmax[c];
+    double u[4];
+    draw4((uint32_t)c, s32, 0xffffffffu, TAG_DSMC, seed, u);
+    const double mr = 0.5 * (double)n * (double)(n - 1) * coeff * gm;
+    const double fl = std::floor(mr);
+    const int64_t M = (int64_t)fl + ((u[0] < mr - fl) ? 1 : 0);
+    for (int64_t k = 0; k < M; ++k) {
+      draw4((uint32_t)c, s32, (uint32_t)k, TAG_DSMC, seed, u);
+      int i = (int)(u[0] * n);
+      i = i > n - 1 ? n - 1 : i;
+      int j = (int)(u[1] * (n - 1));
+      j = j > n - 2 ? n - 2 : j;
+      if (j >= i) ++j;
+      const int64_t pi = b + i, pj = b + j;
+      const double vix = vx[pi], viy = vy[pi], vjx = vx[pj], vjy = vy[pj];
+      const double gx = vix - vjx, gy = viy - vjy;
+      const double g = std::sqrt(gx * gx + gy * gy);
+      if (g > gm) gm = g;
+      if (u[2] * gm < g) {
+        double sn, cs;
+        sincos2pi(u[3], &sn, &cs);
+        const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
+        const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
+        vx[pi] = (float)(cx + hx);
+        vy[pi] = (float)(cy + hy);
+        vx[pj] = (float)(cx - hx);
+        vy[pj] = (float)(cy - hy);
+        ++total;
+      }
+    }
+    gmax[c] = (float)gm;
+  }
+  return total;
+}
+
+void or_sincos2pi(double u, double* s, double* c) { sincos2pi(u, s, c); }
+double or_log(double x) { return log_own(x); }
 
 }  // extern "C"

@@ -135,16 +135,25 @@ void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy,
 /* ---------------- collisions ---------------- */
 /* Analytic cross sections (m^2) at energy E (eV). */
 void or_mcc_cross_sections(double e_ev, double* s_el, double* s_ex, double* s_iz);
-/* Null-collision MCC on one step (mirror of the GPU kernel, FP32 particles).
- * Returns the number of births written to the tail of the arrays (capacity cap).
- * counts[4] receives {elastic, excitation, ionisation, null-or-none}. */
+/* Null-collision MCC pass (mirror of the GPU operator, FP32 particles).  e_per_v2: energy
+ * [eV] per |v|^2; nu_dt_per_sigma_v: nu*dt per (sigma [m^2] * |v|); p_coll = 1 - exp(-nu_max
+ * dt); mass_ratio2 = 2 m_e / M.  Particle p draws Philox {id[p], step, 0|1, 'MCC1'}.  Births
+ * (one per ionisation) are appended after n in ascending parent id with ids next_id, ...
+ * Returns the new particle count; counts[4] += {elastic, excitation, ionisation, null}. */
 int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, float* vy,
-                        int32_t* id, int64_t* next_id, double v_unit, double nu_max_dt_factor,
-                        double ng_sigma_scale, uint64_t seed, int64_t step, int64_t* counts);
-/* NTC hard-sphere DSMC on cell-sorted particles (mirror of the GPU kernel). */
-void or_dsmc_step_f32(int ncells, const int32_t* offsets, const int32_t* id,
-                      float* vx, float* vy, float* gmax, double coll_coeff,
-                      uint64_t seed, int64_t step, int64_t* ncoll);
+                        int32_t* id, int64_t next_id, double e_per_v2,
+                        double nu_dt_per_sigma_v, double p_coll, double nu_max_dt,
+                        double mass_ratio2, uint64_t seed, int64_t step, int64_t* counts);
+/* NTC hard-sphere DSMC pass over cell-sorted particles (mirror of the GPU operator): cell c
+ * holds slots [offsets[c], offsets[c+1]); gmax[c] is the running maximum relative speed;
+ * Philox {c, step, k, 'DSMC'} for candidate k and {c, step, 0xffffffff, 'DSMC'} for the
+ * fractional candidate.  Returns the number of accepted collisions. */
+int64_t or_dsmc_step_f32(int ncells, const int32_t* offsets, float* vx, float* vy,
+                         float* gmax, double coeff, uint64_t seed, int64_t step);
+
+/* The bit-reproducible elementary functions used by the collision stages. */
+void or_sincos2pi(double u, double* s, double* c);
+double or_log(double x);
 
 /* ---------------- adjoint ---------------- */
 /* Loss = sum(Ex^2+Ey^2)*dx*dy after nsteps of the FP64 collisionless run starting from

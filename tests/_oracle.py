@@ -49,6 +49,11 @@ SIG = {
     "or_run_f64": (None, [_I64, _DP, _DP, _DP, _DP, _I, _I, _D, _D, _D, _D, _D, _I, _DP]),
     "or_run_f32": (None, [_I64, _FP, _FP, _FP, _FP, _I, _I, _D, _D, _D, _D, _D, _I, _DP]),
     "or_mcc_cross_sections": (None, [_D, _DP, _DP, _DP]),
+    "or_mcc_step_f32": (_I64, [_I64, _I64, _FP, _FP, _FP, _FP, _IP, _I64, _D, _D, _D, _D, _D,
+                               ctypes.c_uint64, _I64, _LP]),
+    "or_dsmc_step_f32": (_I64, [_I, _IP, _FP, _FP, _FP, _D, ctypes.c_uint64, _I64]),
+    "or_sincos2pi": (None, [_D, _DP, _DP]),
+    "or_log": (_D, [_D]),
 }
 
 
