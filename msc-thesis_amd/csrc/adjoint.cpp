@@ -1,17 +1,327 @@
-// adjoint.cpp -- differentiable collisionless run (C ABI half).
+// adjoint.cpp -- differentiable collisionless run (C ABI half): forward with checkpoints
+// every K steps, reverse sweep by segments with recomputation of the K steps in between.
+// No reference counterpart; the forward step is the reference step order of
+// thesisCode/collisionless/double/src/main.cc:137-172 (deposit, Poisson, E, gather, push).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "adjoint_launch.h"
 #include "engine.h"
+
 using namespace pic;
+
+struct pic_adjoint {
+  pic_config cfg;
+  int fp64 = 0;
+  size_t rs = 4;
+  int64_t n = 0;
+  int nm = 0, nsteps = 0, K = 1, nseg = 0;
+  int device = 0;
+  hipStream_t s = nullptr;
+  AdjGrid g;
+  double qp = 0;
+  // device buffers
+  double *xl = nullptr, *yl = nullptr, *mkx = nullptr, *mky = nullptr;
+  void *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
+  void* ckpt = nullptr;                       // nseg x 4 x n
+  void *trx = nullptr, *try_ = nullptr;       // (K+1) x n
+  void* trE = nullptr;                        // K x padded field
+  double *lx = nullptr, *ly = nullptr, *lvx = nullptr, *lvy = nullptr;
+  double *lE = nullptr, *lrho = nullptr, *rho = nullptr;
+  unsigned long long *hi = nullptr, *lo = nullptr;
+  void* E = nullptr;
+  double2 *S = nullptr, *Sx = nullptr, *Sy = nullptr, *A = nullptr, *tw = nullptr;
+  RunDesc* dd = nullptr;
+  RunState* st = nullptr;
+  int2 *rp = nullptr, *cl = nullptr, *rw = nullptr;
+  int nrp = 0, ncl = 0, nrw = 0;
+  double* part = nullptr;
+  bool forward_done = false;
+  std::vector<void*> all;
+
+  ~pic_adjoint() {
+    (void)hipSetDevice(device);
+    if (s) (void)hipStreamSynchronize(s);
+    for (void* p : all) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  template <typename T>
+  hipError_t get(T** p, size_t count) {
+    hipError_t e = dalloc(p, count);
+    if (e == hipSuccess) all.push_back((void*)*p);
+    return e;
+  }
+  hipError_t getb(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e == hipSuccess) all.push_back(*p);
+    return e;
+  }
+  size_t epad() const { return (size_t)2 * (g.nx + 1) * (g.ny + 1); }  // reals
+  char* at(void* base, size_t elems) const { return (char*)base + elems * rs; }
+};
+
+namespace {
+
+template <typename R>
+int field(pic_adjoint& a) {
+  const long long G = (long long)a.g.nx * a.g.ny;
+  PIC_HIP(hipMemsetAsync(a.hi, 0, sizeof(unsigned long long) * G, a.s));
+  if (a.fp64) PIC_HIP(hipMemsetAsync(a.lo, 0, sizeof(unsigned long long) * G, a.s));
+  launch_op_deposit<R>(a.s, (const R*)a.x, (const R*)a.y, a.n, a.g.nx, a.g.ny, a.g.lx, a.g.ly,
+                       a.hi, a.lo);
+  const double scale = a.qp / (a.g.dx * a.g.dy) * (a.fp64 ? 1.0 : std::ldexp(1.0, -36));
+  launch_op_fixed_to_double(a.s, a.hi, a.lo, G, a.fp64, scale, a.rho);
+  FieldArgs f;
+  std::memset(&f, 0, sizeof(f));
+  f.runs = a.dd; f.st = a.st;
+  f.S = a.S; f.Sx = a.Sx; f.Sy = a.Sy; f.E = a.E; f.phi = nullptr; f.tw = a.tw;
+  f.row_pair_items = a.rp; f.n_row_pair_items = a.nrp;
+  f.col_items = a.cl; f.n_col_items = a.ncl;
+  f.row_items = a.rw; f.n_row_items = a.nrw;
+  f.nruns = 1; f.fp64 = a.fp64; f.want_phi = 0; f.max_nx = a.g.nx; f.max_ny = a.g.ny;
+  f.rho_in = a.rho;
+  launch_field(a.s, f, nullptr);
+  return PIC_OK;
+}
+
+template <typename R>
+int run_steps(pic_adjoint& a, int k0, int k1, bool save_ckpt, bool save_traj) {
+  const size_t n = (size_t)a.n;
+  for (int k = k0; k < k1; ++k) {
+    if (save_ckpt && k % a.K == 0) {
+      char* c = a.at(a.ckpt, (size_t)(k / a.K) * 4 * n);
+      PIC_HIP(hipMemcpyAsync(c, a.x, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+      PIC_HIP(hipMemcpyAsync(c + a.rs * n, a.y, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+      PIC_HIP(hipMemcpyAsync(c + 2 * a.rs * n, a.vx, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+      PIC_HIP(hipMemcpyAsync(c + 3 * a.rs * n, a.vy, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+    }
+    if (save_traj) {
+      PIC_HIP(hipMemcpyAsync(a.at(a.trx, (size_t)(k - k0) * n), a.x, a.rs * n,
+                             hipMemcpyDeviceToDevice, a.s));
+      PIC_HIP(hipMemcpyAsync(a.at(a.try_, (size_t)(k - k0) * n), a.y, a.rs * n,
+                             hipMemcpyDeviceToDevice, a.s));
+    }
+    int rc = field<R>(a);
+    if (rc) return rc;
+    if (save_traj)
+      PIC_HIP(hipMemcpyAsync(a.at(a.trE, (size_t)(k - k0) * a.epad()), a.E, a.rs * a.epad(),
+                             hipMemcpyDeviceToDevice, a.s));
+    adj_push<R>(a.s, a.g, (R*)a.x, (R*)a.y, (R*)a.vx, (R*)a.vy, (const R*)a.E, a.n,
+                k == 0 ? 0.5 : 1.0);
+  }
+  if (save_traj) {
+    PIC_HIP(hipMemcpyAsync(a.at(a.trx, (size_t)(k1 - k0) * n), a.x, a.rs * n,
+                           hipMemcpyDeviceToDevice, a.s));
+    PIC_HIP(hipMemcpyAsync(a.at(a.try_, (size_t)(k1 - k0) * n), a.y, a.rs * n,
+                           hipMemcpyDeviceToDevice, a.s));
+  }
+  PIC_HIP(hipGetLastError());
+  return PIC_OK;
+}
+
+template <typename R>
+int forward_t(pic_adjoint& a, const double* amps, const void* vx0, const void* vy0,
+              double* loss) {
+  adj_init<R>(a.s, a.xl, a.yl, a.n, amps, a.mkx, a.mky, a.nm, a.g.lx, (R*)a.x, (R*)a.y);
+  PIC_HIP(hipMemcpyAsync(a.vx, vx0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
+  PIC_HIP(hipMemcpyAsync(a.vy, vy0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
+  int rc = run_steps<R>(a, 0, a.nsteps, true, false);
+  if (rc) return rc;
+  rc = field<R>(a);
+  if (rc) return rc;
+  const int nb = 256;
+  adj_energy<R>(a.s, a.g, (const R*)a.E, a.part, nb);
+  std::vector<double> h(nb);
+  PIC_HIP(hipMemcpyAsync(h.data(), a.part, sizeof(double) * nb, hipMemcpyDeviceToHost, a.s));
+  PIC_HIP(hipStreamSynchronize(a.s));
+  double sum = 0.0;
+  for (double v : h) sum += v;
+  if (loss) *loss = sum * a.g.dx * a.g.dy;
+  a.forward_done = true;
+  return PIC_OK;
+}
+
+template <typename R>
+int backward_t(pic_adjoint& a, double dloss, double* grad_amps, void* gvx, void* gvy) {
+  const size_t n = (size_t)a.n;
+  const long long G = (long long)a.g.nx * a.g.ny;
+  // x, y hold x_N after the forward: seed lE = 2 dL E_N dx dy, lrho_N = M^T lE
+  int rc = field<R>(a);
+  if (rc) return rc;
+  adj_seed<R>(a.s, a.g, (const R*)a.E, 2.0 * dloss * a.g.dx * a.g.dy, a.lE);
+  adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
+  for (double* p : {a.lx, a.ly, a.lvx, a.lvy})
+    PIC_HIP(hipMemsetAsync(p, 0, sizeof(double) * n, a.s));
+  const double cq = a.qp / (a.g.dx * a.g.dy);
+  for (int seg = a.nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
+    const char* c = a.at(a.ckpt, (size_t)seg * 4 * n);
+    PIC_HIP(hipMemcpyAsync(a.x, c, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+    PIC_HIP(hipMemcpyAsync(a.y, c + a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+    PIC_HIP(hipMemcpyAsync(a.vx, c + 2 * a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+    PIC_HIP(hipMemcpyAsync(a.vy, c + 3 * a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+    if ((rc = run_steps<R>(a, k0, k1, false, true))) return rc;
+    for (int k = k1; k > k0; --k) {
+      PIC_HIP(hipMemsetAsync(a.lE, 0, sizeof(double) * 2 * G, a.s));
+      const double ck1 = (k - 1 == 0 ? 0.5 : 1.0) * a.g.dt * a.g.qm;
+      adj_pass<R>(a.s, a.g, (const R*)a.at(a.trx, (size_t)(k - k0) * n),
+                  (const R*)a.at(a.try_, (size_t)(k - k0) * n), a.lrho, cq,
+                  (const R*)a.at(a.trx, (size_t)(k - 1 - k0) * n),
+                  (const R*)a.at(a.try_, (size_t)(k - 1 - k0) * n),
+                  (const R*)a.at(a.trE, (size_t)(k - 1 - k0) * a.epad()), ck1, a.lx, a.ly,
+                  a.lvx, a.lvy, a.lE, a.n);
+      adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
+    }
+  }
+  // finish step 0 (deposit Jacobian at x_0); trajectory slot 0 of segment 0 holds x_0
+  adj_pass<R>(a.s, a.g, (const R*)a.trx, (const R*)a.try_, a.lrho, cq, nullptr, nullptr, nullptr,
+              0.0, a.lx, a.ly, a.lvx, a.lvy, a.lE, a.n);
+  if (grad_amps)
+    adj_amp_grad(a.s, a.lx, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, 256, 1.0, grad_amps);
+  if (gvx) adj_to_r<R>(a.s, a.lvx, 1.0, (R*)gvx, a.n);
+  if (gvy) adj_to_r<R>(a.s, a.lvy, 1.0, (R*)gvy, a.n);
+  PIC_HIP(hipGetLastError());
+  PIC_HIP(hipStreamSynchronize(a.s));
+  return PIC_OK;
+}
+
+}  // namespace
+
 extern "C" {
-int pic_adjoint_create(const pic_config*, const double*, const double*, int64_t, int,
-                       const int32_t*, const int32_t*, int, int, pic_adjoint**) {
-  return fail(PIC_ERR_UNSUPPORTED, "adjoint not built yet");
+
+int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl, int64_t n,
+                       int nmodes, const int32_t* mx, const int32_t* my, int nsteps,
+                       int checkpoint_every, pic_adjoint** out) {
+  if (!cfg || !xl || !yl || !out || n < 1 || nmodes < 0 || nmodes > 64 || nsteps < 1 ||
+      checkpoint_every < 1 || (nmodes > 0 && (!mx || !my)))
+    return fail(PIC_ERR_ARG, "bad arguments to pic_adjoint_create");
+  int rc = validate_config(*cfg);
+  if (rc) return rc;
+  if (cfg->collide != PIC_COLLIDE_NONE)
+    return fail(PIC_ERR_UNSUPPORTED, "the adjoint covers the collisionless step only");
+  *out = nullptr;
+  std::unique_ptr<pic_adjoint> a(new pic_adjoint());
+  a->cfg = *cfg;
+  a->device = cfg->device;
+  PIC_HIP(hipSetDevice(a->device));
+  a->fp64 = cfg->precision == PIC_FP64;
+  a->rs = a->fp64 ? 8 : 4;
+  a->n = n;
+  a->nm = nmodes;
+  a->nsteps = nsteps;
+  a->K = checkpoint_every;
+  a->nseg = (nsteps + a->K - 1) / a->K;
+  int l = 0;
+  AdjGrid& g = a->g;
+  g.nx = cfg->nx; g.ny = cfg->ny;
+  while ((1 << l) < g.nx) ++l;
+  g.lognx = l;
+  l = 0;
+  while ((1 << l) < g.ny) ++l;
+  g.logny = l;
+  g.lx = cfg->lx; g.ly = cfg->ly; g.dx = g.lx / g.nx; g.dy = g.ly / g.ny;
+  g.dt = cfg->dt; g.qm = cfg->qm;
+  a->qp = cfg->qp != 0.0 ? cfg->qp : -(cfg->lx * cfg->ly) / (double)n;
+  PIC_HIP(hipStreamCreateWithFlags(&a->s, hipStreamNonBlocking));
+  const size_t G = (size_t)g.nx * g.ny;
+  pic_adjoint& A = *a;
+  PIC_HIP(A.get(&A.xl, n));
+  PIC_HIP(A.get(&A.yl, n));
+  PIC_HIP(A.get(&A.mkx, std::max(nmodes, 1)));
+  PIC_HIP(A.get(&A.mky, std::max(nmodes, 1)));
+  PIC_HIP(A.getb(&A.x, A.rs * n));
+  PIC_HIP(A.getb(&A.y, A.rs * n));
+  PIC_HIP(A.getb(&A.vx, A.rs * n));
+  PIC_HIP(A.getb(&A.vy, A.rs * n));
+  PIC_HIP(A.getb(&A.ckpt, A.rs * 4 * n * A.nseg));
+  PIC_HIP(A.getb(&A.trx, A.rs * n * (A.K + 1)));
+  PIC_HIP(A.getb(&A.try_, A.rs * n * (A.K + 1)));
+  PIC_HIP(A.getb(&A.trE, A.rs * A.epad() * A.K));
+  PIC_HIP(A.get(&A.lx, n));
+  PIC_HIP(A.get(&A.ly, n));
+  PIC_HIP(A.get(&A.lvx, n));
+  PIC_HIP(A.get(&A.lvy, n));
+  PIC_HIP(A.get(&A.lE, 2 * G));
+  PIC_HIP(A.get(&A.lrho, G));
+  PIC_HIP(A.get(&A.rho, G));
+  PIC_HIP(A.get(&A.hi, G));
+  PIC_HIP(A.get(&A.lo, G));
+  PIC_HIP(A.getb(&A.E, A.rs * A.epad()));
+  const size_t ss = (size_t)(g.ny / 2 + 1) * g.nx;
+  PIC_HIP(A.get(&A.S, ss));
+  PIC_HIP(A.get(&A.Sx, ss));
+  PIC_HIP(A.get(&A.Sy, ss));
+  PIC_HIP(A.get(&A.A, G));
+  PIC_HIP(A.get(&A.tw, PIC_FFT_MAX / 2));
+  PIC_HIP(A.get(&A.dd, 1));
+  PIC_HIP(A.get(&A.st, 1));
+  PIC_HIP(A.get(&A.part, 256 * std::max(nmodes, 1)));
+  PIC_HIP(hipMemcpy(A.xl, xl, sizeof(double) * n, hipMemcpyHostToDevice));
+  PIC_HIP(hipMemcpy(A.yl, yl, sizeof(double) * n, hipMemcpyHostToDevice));
+  std::vector<double> kx(std::max(nmodes, 1), 0.0), ky(std::max(nmodes, 1), 0.0);
+  const double twopi = 6.283185307179586476925;
+  for (int m = 0; m < nmodes; ++m) {
+    kx[m] = twopi * mx[m] / cfg->lx;
+    ky[m] = twopi * my[m] / cfg->ly;
+  }
+  PIC_HIP(hipMemcpy(A.mkx, kx.data(), sizeof(double) * kx.size(), hipMemcpyHostToDevice));
+  PIC_HIP(hipMemcpy(A.mky, ky.data(), sizeof(double) * ky.size(), hipMemcpyHostToDevice));
+  std::vector<double2> h(PIC_FFT_MAX / 2);
+  for (int j = 0; j < PIC_FFT_MAX / 2; ++j) {
+    const long double ang = 2.0L * 3.141592653589793238462643383279L * j / PIC_FFT_MAX;
+    h[j] = make_double2((double)cosl(ang), (double)-sinl(ang));
+  }
+  PIC_HIP(hipMemcpy(A.tw, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+  RunDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.nx = g.nx; d.ny = g.ny; d.lognx = g.lognx; d.logny = g.logny;
+  d.lx = g.lx; d.ly = g.ly; d.dx = g.dx; d.dy = g.dy;
+  d.row_item_begin = 0; d.row_item_end = g.nx;
+  PIC_HIP(hipMemcpy(A.dd, &d, sizeof(d), hipMemcpyHostToDevice));
+  PIC_HIP(hipMemset(A.st, 0, sizeof(RunState)));
+  std::vector<int2> rp, cl, rw;
+  for (int i = 0; i < g.nx; i += 2) rp.push_back(make_int2(0, i));
+  for (int k = 0; k <= g.ny / 2; ++k) cl.push_back(make_int2(0, k));
+  for (int i = 0; i < g.nx; ++i) rw.push_back(make_int2(0, i));
+  PIC_HIP(A.get(&A.rp, rp.size()));
+  PIC_HIP(A.get(&A.cl, cl.size()));
+  PIC_HIP(A.get(&A.rw, rw.size()));
+  PIC_HIP(hipMemcpy(A.rp, rp.data(), sizeof(int2) * rp.size(), hipMemcpyHostToDevice));
+  PIC_HIP(hipMemcpy(A.cl, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice));
+  PIC_HIP(hipMemcpy(A.rw, rw.data(), sizeof(int2) * rw.size(), hipMemcpyHostToDevice));
+  A.nrp = (int)rp.size(); A.ncl = (int)cl.size(); A.nrw = (int)rw.size();
+  *out = a.release();
+  return PIC_OK;
 }
-int pic_adjoint_destroy(pic_adjoint*) { return PIC_OK; }
-int pic_adjoint_forward(pic_adjoint*, const double*, const void*, const void*, double*) {
-  return fail(PIC_ERR_UNSUPPORTED, "adjoint not built yet");
+
+int pic_adjoint_destroy(pic_adjoint* adj) {
+  delete adj;
+  return PIC_OK;
 }
-int pic_adjoint_backward(pic_adjoint*, double, double*, void*, void*) {
-  return fail(PIC_ERR_UNSUPPORTED, "adjoint not built yet");
+
+int pic_adjoint_forward(pic_adjoint* a, const double* amps, const void* vx0, const void* vy0,
+                        double* loss) {
+  if (!a || !vx0 || !vy0 || (a->nm > 0 && !amps)) return fail(PIC_ERR_ARG, "bad arguments");
+  PIC_HIP(hipSetDevice(a->device));
+  return a->fp64 ? forward_t<double>(*a, amps, vx0, vy0, loss)
+                 : forward_t<float>(*a, amps, vx0, vy0, loss);
 }
-int pic_adjoint_stream(pic_adjoint*, void**) { return fail(PIC_ERR_UNSUPPORTED, "adjoint not built yet"); }
+
+int pic_adjoint_backward(pic_adjoint* a, double dloss, double* grad_amps, void* grad_vx,
+                         void* grad_vy) {
+  if (!a) return fail(PIC_ERR_ARG, "NULL adjoint");
+  if (!a->forward_done) return fail(PIC_ERR_STATE, "pic_adjoint_backward before forward");
+  PIC_HIP(hipSetDevice(a->device));
+  return a->fp64 ? backward_t<double>(*a, dloss, grad_amps, grad_vx, grad_vy)
+                 : backward_t<float>(*a, dloss, grad_amps, grad_vx, grad_vy);
 }
+
+int pic_adjoint_stream(pic_adjoint* a, void** stream) {
+  if (!a || !stream) return fail(PIC_ERR_ARG, "NULL argument");
+  *stream = (void*)a->s;
+  return PIC_OK;
+}
+
+}  // extern "C"

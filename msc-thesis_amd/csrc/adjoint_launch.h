@@ -1,0 +1,31 @@
+// adjoint_launch.h -- launchers of the adjoint kernels (internal to libpicamd.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct AdjGrid {
+  int nx, ny, lognx, logny;
+  double lx, ly, dx, dy, dt, qm;
+};
+
+template <typename R>
+void adj_init(hipStream_t s, const double* xl, const double* yl, int64_t n, const double* amps,
+              const double* mkx, const double* mky, int nm, double lx, R* x, R* y);
+template <typename R>
+void adj_push(hipStream_t s, const AdjGrid& g, R* x, R* y, R* vx, R* vy, const R* E, int64_t n,
+              double c);
+template <typename R>
+void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const double* lrho,
+              double cq, const R* xp, const R* yp, const R* Eprev, double ck1, double* lx,
+              double* ly, double* lvx, double* lvy, double* lE, int64_t n);
+template <typename R>
+void adj_seed(hipStream_t s, const AdjGrid& g, const R* E, double sc, double* lE);
+template <typename R>
+void adj_energy(hipStream_t s, const AdjGrid& g, const R* E, double* part, int nblocks);
+void adj_mt(hipStream_t s, const AdjGrid& g, const double* lE, double2* A, double* lrho,
+            const double2* tw);
+void adj_amp_grad(hipStream_t s, const double* lx, const double* xl, const double* yl, int64_t n,
+                  const double* mkx, const double* mky, int nm, double* part, int nblocks,
+                  double scale, double* out);
+template <typename R>
+void adj_to_r(hipStream_t s, const double* a, double sc, R* out, int64_t n);

@@ -12,35 +12,10 @@
 //   field   : Ex + i Ey from one inverse complex row FFT per row; writes the padded
 //             (Ex, Ey) node array and the field energy (fixed-order reduction)
 #include <type_traits>
+#include "fft_lds.h"
 #include "launch.h"
 
 namespace {
-
-__device__ __forceinline__ int bitrev(int v, int logn) {
-  return (int)(__brev((unsigned)v) >> (32 - logn));
-}
-
-// radix-2 DIT on LDS data stored in bit-reversed order; sign -1 forward, +1 inverse
-__device__ void fft_lds(double2* a, int n, int logn, int sign, const double2* __restrict__ tw) {
-  const double ws = (sign < 0) ? 1.0 : -1.0;
-  for (int s = 1; s <= logn; ++s) {
-    const int h = 1 << (s - 1);
-    const int ts = PIC_FFT_MAX >> s;
-    for (int b = threadIdx.x; b < (n >> 1); b += PIC_WG) {
-      const int j = b & (h - 1);
-      const int i0 = ((b >> (s - 1)) << s) + j;
-      const int i1 = i0 + h;
-      double2 w = tw[j * ts];
-      w.y *= ws;
-      const double2 u = a[i0], v = a[i1];
-      const double tr = fma(v.x, w.x, -v.y * w.y);
-      const double ti = fma(v.x, w.y, v.y * w.x);
-      a[i0] = make_double2(u.x + tr, u.y + ti);
-      a[i1] = make_double2(u.x - tr, u.y - ti);
-    }
-    __syncthreads();
-  }
-}
 
 __device__ __forceinline__ double fixed_to_density(unsigned long long hi, unsigned long long lo,
                                                    int fp64, double scale) {
@@ -48,14 +23,6 @@ __device__ __forceinline__ double fixed_to_density(unsigned long long hi, unsign
     return ((double)(long long)hi * 9.313225746154785e-10 +
             (double)(long long)lo * 8.673617379884035e-19) * scale;
   return (double)(long long)hi * scale;
-}
-
-// sin(2 pi m / n) from the twiddle table (tw[j] = exp(-2 pi i j / PIC_FFT_MAX))
-__device__ __forceinline__ double sin2pi_frac(int m, int n, const double2* __restrict__ tw) {
-  const int st = PIC_FFT_MAX / n;
-  if (m == 0 || 2 * m == n) return 0.0;
-  if (2 * m < n) return -tw[m * st].y;
-  return tw[(n - m) * st].y;
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {

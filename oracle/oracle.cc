@@ -820,6 +820,141 @@ int64_t or_dsmc_step_f32(int ncells, const int32_t* offsets, float* vx, float* v
   return total;
 }
 
+// ---------------------------------------------------------------- adjoint
+// Reverse mode of the continuous-weight FP64 run, with the whole trajectory kept in memory
+// (small cases).  E = M rho with M = -Dc P (P: spectral Green operator, symmetric; Dc:
+// periodic central difference, antisymmetric), so M^T (lx, ly) = P (Dcx lx + Dcy ly).
+void or_adjoint_mt(int nx, int ny, double lx, double ly, const double* lex, const double* ley,
+                   double* lrho) {
+  const size_t G = (size_t)nx * ny;
+  std::vector<double> ax(G), ay(G), bx(G), by(G), g(G);
+  efield_impl(nx, ny, lx, ly, lex, ax.data(), ay.data());  // ax = -Dcx lex
+  efield_impl(nx, ny, lx, ly, ley, bx.data(), by.data());  // by = -Dcy ley
+  for (size_t i = 0; i < G; ++i) g[i] = -(ax[i] + by[i]);
+  poisson_impl(nx, ny, lx, ly, g.data(), lrho, nullptr);
+}
+
+namespace {
+struct Stencil {
+  size_t nd[4];
+  double w[4], wx[4], wy[4];
+};
+Stencil stencil(double x, double y, int nx, int ny, double dx, double dy) {
+  int cx, cy;
+  double fx, fy;
+  locate(x, dx, nx, &cx, &fx);
+  locate(y, dy, ny, &cy, &fy);
+  const int cxp = wrapi(cx + 1, nx), cyp = wrapi(cy + 1, ny);
+  Stencil s;
+  s.nd[0] = (size_t)cx * ny + cy;
+  s.nd[1] = (size_t)cxp * ny + cy;
+  s.nd[2] = (size_t)cx * ny + cyp;
+  s.nd[3] = (size_t)cxp * ny + cyp;
+  s.w[0] = (1 - fx) * (1 - fy); s.w[1] = fx * (1 - fy); s.w[2] = (1 - fx) * fy; s.w[3] = fx * fy;
+  s.wx[0] = -(1 - fy) / dx; s.wx[1] = (1 - fy) / dx; s.wx[2] = -fy / dx; s.wx[3] = fy / dx;
+  s.wy[0] = -(1 - fx) / dy; s.wy[1] = -fx / dy; s.wy[2] = (1 - fx) / dy; s.wy[3] = fx / dy;
+  return s;
+}
+}  // namespace
+
+double or_adjoint_f64(int64_t n, const double* xl, const double* yl, const double* vx0,
+                      const double* vy0, int nmodes, const int32_t* mx, const int32_t* my,
+                      const double* amp, int nx, int ny, double lx, double ly, double dt,
+                      double qm, double qp, int nsteps, double* grad_a, double* grad_vx,
+                      double* grad_vy) {
+  const size_t G = (size_t)nx * ny, N = (size_t)n;
+  const double dx = lx / nx, dy = ly / ny, dA = dx * dy;
+  std::vector<double> kx(nmodes), ky(nmodes);
+  for (int m = 0; m < nmodes; ++m) {
+    kx[m] = 2.0 * kPi * mx[m] / lx;
+    ky[m] = 2.0 * kPi * my[m] / ly;
+  }
+  // forward with the trajectory: X[k], Y[k] for k = 0..nsteps, EX/EY[k] for k = 0..nsteps
+  std::vector<double> X((nsteps + 1) * N), Y((nsteps + 1) * N);
+  std::vector<double> EX((nsteps + 1) * G), EY((nsteps + 1) * G);
+  std::vector<double> vx(vx0, vx0 + N), vy(vy0, vy0 + N), rho(G), phi(G);
+  for (size_t p = 0; p < N; ++p) {
+    double d = 0.0;
+    for (int m = 0; m < nmodes; ++m) d += amp[m] * std::cos(kx[m] * xl[p] + ky[m] * yl[p]);
+    X[p] = wrap_pos(xl[p] + d, lx);
+    Y[p] = yl[p];
+  }
+  for (int k = 0; k <= nsteps; ++k) {
+    double* x = &X[k * N];
+    double* y = &Y[k * N];
+    double* ex = &EX[k * G];
+    double* ey = &EY[k * G];
+    or_deposit_f64(n, x, y, nx, ny, lx, ly, qp, rho.data());
+    poisson_impl(nx, ny, lx, ly, rho.data(), phi.data(), nullptr);
+    efield_impl(nx, ny, lx, ly, phi.data(), ex, ey);
+    if (k == nsteps) break;
+    std::memcpy(&X[(k + 1) * N], x, sizeof(double) * N);
+    std::memcpy(&Y[(k + 1) * N], y, sizeof(double) * N);
+    push_cont(n, &X[(k + 1) * N], &Y[(k + 1) * N], vx.data(), vy.data(), ex, ey, nx, ny, lx, ly,
+              dt, qm, k == 0 ? 0.5 : 1.0, 1, nullptr, nullptr, nullptr);
+  }
+  const double* exN = &EX[(size_t)nsteps * G];
+  const double* eyN = &EY[(size_t)nsteps * G];
+  double loss = 0.0;
+  for (size_t i = 0; i < G; ++i) loss += exN[i] * exN[i] + eyN[i] * eyN[i];
+  loss *= dA;
+  if (!grad_a && !grad_vx && !grad_vy) return loss;
+  // reverse sweep
+  std::vector<double> ax(N, 0.0), ay(N, 0.0), bx(N, 0.0), by(N, 0.0);
+  std::vector<double> lex(G), ley(G), lrho(G);
+  for (size_t i = 0; i < G; ++i) { lex[i] = 2.0 * dA * exN[i]; ley[i] = 2.0 * dA * eyN[i]; }
+  or_adjoint_mt(nx, ny, lx, ly, lex.data(), ley.data(), lrho.data());
+  const double cq = qp / dA;
+  for (int k = nsteps;; --k) {
+    // deposit Jacobian at x_k
+    const double* x = &X[(size_t)k * N];
+    const double* y = &Y[(size_t)k * N];
+    for (size_t p = 0; p < N; ++p) {
+      const Stencil s = stencil(x[p], y[p], nx, ny, dx, dy);
+      for (int q = 0; q < 4; ++q) {
+        ax[p] += cq * s.wx[q] * lrho[s.nd[q]];
+        ay[p] += cq * s.wy[q] * lrho[s.nd[q]];
+      }
+    }
+    if (k == 0) break;
+    // step k-1: drift, then kick at x_{k-1} with E_{k-1}
+    const int j = k - 1;
+    const double ck = (j == 0 ? 0.5 : 1.0) * dt * qm;
+    const double* xp = &X[(size_t)j * N];
+    const double* yp = &Y[(size_t)j * N];
+    const double* ex = &EX[(size_t)j * G];
+    const double* ey = &EY[(size_t)j * G];
+    std::fill(lex.begin(), lex.end(), 0.0);
+    std::fill(ley.begin(), ley.end(), 0.0);
+    for (size_t p = 0; p < N; ++p) {
+      bx[p] += dt * ax[p];
+      by[p] += dt * ay[p];
+      const Stencil s = stencil(xp[p], yp[p], nx, ny, dx, dy);
+      double gxx = 0, gxy = 0, gyx = 0, gyy = 0;
+      for (int q = 0; q < 4; ++q) {
+        lex[s.nd[q]] += ck * s.w[q] * bx[p];
+        ley[s.nd[q]] += ck * s.w[q] * by[p];
+        gxx += s.wx[q] * ex[s.nd[q]];
+        gxy += s.wy[q] * ex[s.nd[q]];
+        gyx += s.wx[q] * ey[s.nd[q]];
+        gyy += s.wy[q] * ey[s.nd[q]];
+      }
+      ax[p] += ck * (bx[p] * gxx + by[p] * gyx);
+      ay[p] += ck * (bx[p] * gxy + by[p] * gyy);
+    }
+    or_adjoint_mt(nx, ny, lx, ly, lex.data(), ley.data(), lrho.data());
+  }
+  if (grad_a)
+    for (int m = 0; m < nmodes; ++m) {
+      double s = 0.0;
+      for (size_t p = 0; p < N; ++p) s += ax[p] * std::cos(kx[m] * xl[p] + ky[m] * yl[p]);
+      grad_a[m] = s;
+    }
+  if (grad_vx) std::memcpy(grad_vx, bx.data(), sizeof(double) * N);
+  if (grad_vy) std::memcpy(grad_vy, by.data(), sizeof(double) * N);
+  return loss;
+}
+
 void or_sincos2pi(double u, double* s, double* c) { sincos2pi(u, s, c); }
 double or_log(double x) { return log_own(x); }
 

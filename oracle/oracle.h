@@ -156,6 +156,10 @@ void or_sincos2pi(double u, double* s, double* c);
 double or_log(double x);
 
 /* ---------------- adjoint ---------------- */
+/* Transpose of the field map rho -> (Ex, Ey) (spectral Poisson + central difference):
+ * lrho = M^T (lex, ley). */
+void or_adjoint_mt(int nx, int ny, double lx, double ly, const double* lex, const double* ley,
+                   double* lrho);
 /* Loss = sum(Ex^2+Ey^2)*dx*dy after nsteps of the FP64 collisionless run starting from
  * x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl), v0.  Returns loss; if grad_a / grad_vx /
  * grad_vy are non-NULL the reverse-mode gradient is written there (nmodes, n, n). */

@@ -54,6 +54,9 @@ SIG = {
     "or_dsmc_step_f32": (_I64, [_I, _IP, _FP, _FP, _FP, _D, ctypes.c_uint64, _I64]),
     "or_sincos2pi": (None, [_D, _DP, _DP]),
     "or_log": (_D, [_D]),
+    "or_adjoint_mt": (None, [_I, _I, _D, _D, _DP, _DP, _DP]),
+    "or_adjoint_f64": (_D, [_I64, _DP, _DP, _DP, _DP, _I, _IP, _IP, _DP, _I, _I, _D, _D, _D,
+                            _D, _D, _I, _DP, _DP, _DP]),
 }
 
 
@@ -182,6 +185,34 @@ class Oracle:
         self.lib.or_run_f32(x.size, self.p(x), self.p(y), self.p(vx), self.p(vy), nx, ny, lx,
                             ly, dt, qm, qp, nsteps, self.p(diag))
         return diag
+
+    def adjoint_mt(self, lex, ley, lx, ly):
+        nx, ny = lex.shape
+        lex = np.ascontiguousarray(lex, dtype=np.float64)
+        ley = np.ascontiguousarray(ley, dtype=np.float64)
+        out = np.empty((nx, ny))
+        self.lib.or_adjoint_mt(nx, ny, lx, ly, self.p(lex), self.p(ley), self.p(out))
+        return out
+
+    def adjoint_f64(self, xl, yl, vx0, vy0, modes, amps, nx, ny, lx, ly, dt, qm, qp, nsteps,
+                    grad=True):
+        """(loss, grad_amps, grad_vx, grad_vy); gradients None when grad is False."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        xl, yl, vx0, vy0, amps = f(xl), f(yl), f(vx0), f(vy0), f(amps)
+        modes = np.asarray(modes, dtype=np.int32).reshape(-1, 2)
+        mx = np.ascontiguousarray(modes[:, 0])
+        my = np.ascontiguousarray(modes[:, 1])
+        n, nm = xl.size, modes.shape[0]
+        ga = np.empty(max(nm, 1)) if grad else None
+        gx = np.empty(n) if grad else None
+        gy = np.empty(n) if grad else None
+        loss = self.lib.or_adjoint_f64(n, self.p(xl), self.p(yl), self.p(vx0), self.p(vy0),
+                                       nm, self.p(mx), self.p(my), self.p(amps), nx, ny, lx,
+                                       ly, dt, qm, qp, nsteps, self.p(ga), self.p(gx),
+                                       self.p(gy))
+        if not grad:
+            return loss, None, None, None
+        return loss, ga[:nm], gx, gy
 
 
 def build():
