@@ -7,7 +7,7 @@ sort).  Inputs are resident in HBM when the timed region starts.
 
 Prints ONE JSON line.  Extra keys: roofline (dominant kernel, HIP-event timed), cpu_baseline
 (the CPU oracle on a bounded sample), stages (ms per step per stage), batched_vs_sequential
-(48-run ragged ensemble) and adjoint_over_forward when requested.
+(48-run ragged ensemble) and adjoint_over_forward (128^2 x 32 ppc, 100 steps).
 """
 import argparse
 import json
@@ -31,8 +31,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble comparison")
-    ap.add_argument("--adjoint", action="store_true", help="also time the adjoint")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed tile,halo,splits,sort_every instead of the autotuner")
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--ppc", type=int, default=64)
     return ap.parse_args()
@@ -108,19 +112,28 @@ def batch_ratio(steps=300):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
-              "multi-GPU sharding is a design in DESIGN.md, not a measured path.",
-              file=sys.stderr)
+        print(f"bench.py: the engine runs on one GPU only (got --gpus {args.gpus}); "
+              "sharding across GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     import torch
     from pic_amd import Simulation, presets
 
     torch.cuda.init()
+    if args.copy_only:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}))
+        return
     cfg, parts, _ = presets.landau_512(ppc=args.ppc, n=args.grid)
     n = parts[0].size
     sim = Simulation(cfg)
     sim.set_particles(*parts)
-    if not args.no_autotune:
+    if args.tuning:
+        sim.set_tuning(*(int(v) for v in args.tuning.split(",")))
+    elif not args.no_autotune:
         sim.autotune(trial_steps=8)
     tune = sim.tuning()
     sim.step(args.warmup)
@@ -169,13 +182,13 @@ def main():
         "algorithmic_roofline_frac": value * (40.0 + 72.0 * G / n) / (HBM_PEAK_GBS * 1e9),
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(parts, cfg, 10)
+        line["cpu_baseline"] = cpu_baseline(parts, cfg, 40)
     if not args.no_batch:
         try:
             line["batched_vs_sequential"] = batch_ratio()
         except Exception as e:  # reported, not hidden
             line["batched_vs_sequential"] = {"error": repr(e)}
-    if args.adjoint:
+    if not args.no_adjoint:
         try:
             from pic_amd.adjoint import time_adjoint
             line["adjoint_over_forward"] = time_adjoint()

@@ -40,6 +40,7 @@ void ref_force(int n, const double* x, const double* y, int xdim, int ydim, int 
     g.electricField[i] = Point{ex_pad[i], ey_pad[i]};
   for (int i = 0; i < n; ++i) {
     Particle q(Point{x[i], y[i]}, p);
+    q.setVelocity(Point{0.0, 0.0});  // the constructor leaves the velocity unset
     q.forceCalculation(g, p);
     q.positionUpdate(p);  // velocity 0, forceOld 0: only v = dt/2 * force changes
     Point v = q.getVelocity();

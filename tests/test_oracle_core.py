@@ -1,0 +1,287 @@
+"""Pins of the CPU oracle (no GPU): known-answer vectors, brute force, closed forms,
+conservation laws and the plasma physics of the whole step.  Each test fixes a property a
+plausible mistake in the restatement (dropped term, sign, index, transposed operand) breaks."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+FOUR_PI = 4.0 * math.pi
+
+
+def u32(*v):
+    return np.array(v, dtype=np.uint32)
+
+
+# ------------------------------------------------------------------ random numbers
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox4x32_10_known_answers(oracle, ctr, key, expect):
+    """Random123 known-answer vectors for Philox4x32-10 (Salmon et al., SC'11)."""
+    out = np.zeros(4, np.uint32)
+    oracle.or_philox4x32_10(oracle.p(u32(*ctr)), oracle.p(u32(*key)), oracle.p(out))
+    assert tuple(int(v) for v in out) == expect
+
+
+def test_std_engine_known_answers(oracle):
+    """ISO C++ [rand.predef]: the 10000th output of a default-constructed engine."""
+    assert oracle.or_minstd_rand0_nth(1, 10000) == 1043618065
+    assert oracle.or_mt19937_nth(5489, 10000) == 4123659995
+
+
+# ------------------------------------------------------------------ cell keys and sort
+def edge_positions(rng, n, L, m):
+    d = L / m
+    x = rng.uniform(0, L, n)
+    nodes = np.arange(m) * d
+    x[: m] = nodes                                  # exactly on cell edges
+    x[m: 2 * m] = np.nextafter(nodes + d, 0)        # just below the next edge
+    x[2 * m] = np.nextafter(L, 0)                   # largest value below L
+    x[2 * m + 1] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_cell_keys_brute_force(oracle, dtype):
+    rng = np.random.default_rng(1)
+    nx, ny, lx, ly = 32, 16, FOUR_PI, 5.0
+    n = 5000
+    x = edge_positions(rng, n, lx, nx).astype(dtype)
+    y = edge_positions(rng, n, ly, ny).astype(dtype)
+    k = oracle.cell_keys(x, y, nx, ny, lx, ly)
+    dx, dy = dtype(lx / nx), dtype(ly / ny)
+    cx = np.minimum(np.trunc(x / dx).astype(np.int64), nx - 1)
+    cy = np.minimum(np.trunc(y / dy).astype(np.int64), ny - 1)
+    assert np.array_equal(k, (cx * ny + cy).astype(np.int32))
+
+
+@pytest.mark.parametrize("n,ncells", [(0, 7), (1, 1), (1000, 17), (20000, 4096), (5000, 3)])
+def test_stable_sort_brute_force(oracle, n, ncells):
+    rng = np.random.default_rng(n + ncells)
+    keys = rng.integers(0, max(ncells // 2, 1), n).astype(np.int32)  # leaves empty cells
+    perm, cnt, off = oracle.stable_sort(keys, ncells)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable").astype(np.int32))
+    assert np.array_equal(cnt, np.bincount(keys, minlength=ncells).astype(np.int32))
+    assert off[0] == 0 and off[-1] == n
+    assert np.array_equal(np.diff(off), cnt)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+
+
+# ------------------------------------------------------------------ deposit
+def test_deposit_partition_of_unity_and_stencils(oracle):
+    nx, ny, lx, ly = 8, 4, 8.0, 2.0
+    dx, dy = lx / nx, ly / ny
+    qp = -0.75
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(0, lx, 1000), rng.uniform(0, ly, 1000)
+    rho = oracle.deposit_f64(x, y, nx, ny, lx, ly, qp)
+    assert abs(rho.sum() * dx * dy - 1000 * qp) < 1e-11
+    for fx, fy in [(0, 0), (0.25, 0.5), (0.5, 0.5), (0.75, 0.25)]:
+        cx, cy = nx - 1, 1                      # the x stencil wraps to node 0
+        px, py = np.array([(cx + fx) * dx]), np.array([(cy + fy) * dy])
+        rho = oracle.deposit_f64(px, py, nx, ny, lx, ly, 1.0) * dx * dy
+        want = np.zeros((nx, ny))
+        want[cx, cy] += (1 - fx) * (1 - fy)
+        want[0, cy] += fx * (1 - fy)
+        want[cx, cy + 1] += (1 - fx) * fy
+        want[0, cy + 1] += fx * fy
+        assert np.allclose(rho, want, atol=1e-15)
+
+
+def test_fixed_point_deposit_exact_sums(oracle):
+    nx, ny, lx, ly = 16, 16, FOUR_PI, FOUR_PI
+    rng = np.random.default_rng(4)
+    n = 3000
+    x, y = rng.uniform(0, lx, n), rng.uniform(0, ly, n)
+    acc = oracle.deposit_fixed(x.astype(np.float32), y.astype(np.float32), nx, ny, lx, ly)
+    assert int(acc.sum()) == n << 36
+    hi, lo = oracle.deposit_fixed(x, y, nx, ny, lx, ly)
+    assert (int(hi.sum()) << 30) + int(lo.sum()) == n << 60
+    # a particle at a cell centre: a quarter of the unit on each of the four nodes
+    c = np.array([2.5 * lx / nx], np.float32), np.array([3.5 * ly / ny], np.float32)
+    a1 = oracle.deposit_fixed(c[0], c[1], nx, ny, lx, ly)
+    assert sorted(a1[a1 != 0].tolist()) == [1 << 34] * 4
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 2.0 ** -17), (np.float64, 2.0 ** -29)])
+def test_fixed_point_charge_close_to_continuous(oracle, dtype, tol):
+    nx, ny, lx, ly = 16, 8, 6.0, 3.0
+    rng = np.random.default_rng(5)
+    n = 2000
+    x, y = rng.uniform(0, lx, n), rng.uniform(0, ly, n)
+    qp = -(lx * ly) / n
+    acc = oracle.deposit_fixed(x.astype(dtype), y.astype(dtype), nx, ny, lx, ly)
+    rho_q = oracle.fixed_to_rho(acc, nx, ny, lx, ly, qp)
+    rho_c = oracle.deposit_f64(x.astype(dtype).astype(np.float64),
+                               y.astype(dtype).astype(np.float64), nx, ny, lx, ly, qp)
+    # each weight is quantised to 2^-QB per axis: error per particle <= ~2 * 2^-QB
+    per_node = np.bincount(oracle.cell_keys(x, y, nx, ny, lx, ly), minlength=nx * ny).max()
+    bound = 4 * per_node * tol * abs(qp) / (lx / nx * ly / ny)
+    assert np.abs(rho_q - rho_c).max() <= bound
+
+
+# ------------------------------------------------------------------ FFT, Poisson, E
+@pytest.mark.parametrize("nx,ny", [(8, 4), (16, 16), (4, 32)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_brute_force_dft_and_numpy(oracle, nx, ny, sign):
+    rng = np.random.default_rng(nx * ny)
+    z = rng.normal(size=(nx, ny)) + 1j * rng.normal(size=(nx, ny))
+    buf = np.ascontiguousarray(np.stack([z.real, z.imag], -1).reshape(-1))
+    brute = np.zeros_like(buf)
+    oracle.or_dft2_brute(nx, ny, oracle.p(buf), oracle.p(brute), sign)
+    oracle.or_fft2(nx, ny, oracle.p(buf), sign)
+    got = buf[0::2] + 1j * buf[1::2]
+    want = (np.fft.fft2(z) if sign < 0 else np.fft.ifft2(z) * nx * ny).reshape(-1)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    assert np.abs(got - (brute[0::2] + 1j * brute[1::2])).max() < 1e-11 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("mx,my", [(1, 0), (0, 3), (2, 5), (7, 1)])
+def test_poisson_single_mode_closed_form(oracle, mx, my):
+    nx, ny, lx, ly = 32, 16, 5.0, 2.5
+    X = np.arange(nx)[:, None] * lx / nx
+    Y = np.arange(ny)[None, :] * ly / ny
+    kx, ky = 2 * math.pi * mx / lx, 2 * math.pi * my / ly
+    rho = np.cos(kx * X + ky * Y) + 0.3          # the mean is removed (k = 0 -> 0)
+    phi = oracle.poisson(rho, lx, ly)
+    assert np.allclose(phi, np.cos(kx * X + ky * Y) / (kx * kx + ky * ky), atol=1e-13)
+
+
+def test_efield_closed_form(oracle):
+    nx, ny, lx, ly = 32, 16, 5.0, 2.5
+    dx, dy = lx / nx, ly / ny
+    X = np.arange(nx)[:, None] * dx * np.ones((1, ny))
+    Y = np.arange(ny)[None, :] * dy * np.ones((nx, 1))
+    k, q = 2 * math.pi * 3 / lx, 2 * math.pi * 2 / ly
+    ex, ey = oracle.efield(np.sin(k * X) + np.cos(q * Y), lx, ly)
+    # E = -grad phi by periodic central differences: -sin(k dx)/dx cos(kx), +sin(q dy)/dy sin(qy)
+    assert np.allclose(ex, -math.sin(k * dx) / dx * np.cos(k * X), atol=1e-13)
+    assert np.allclose(ey, math.sin(q * dy) / dy * np.sin(q * Y), atol=1e-13)
+
+
+# ------------------------------------------------------------------ gather / push
+def test_constant_field_orbit_is_exact(oracle):
+    """Leapfrog is exact for a constant force: x_n = x0 + v0 t + a t^2 / 2."""
+    nx, ny, lx, ly = 8, 8, 1000.0, 1000.0
+    E0x, E0y, qm, dt = 0.3, -0.2, -1.0, 0.1
+    ex, ey = np.full((nx, ny), E0x), np.full((nx, ny), E0y)
+    x0 = np.array([400.0, 512.25]); y0 = np.array([300.0, 611.5])
+    vx0 = np.array([1.0, -2.0]); vy0 = np.array([0.5, 0.0])
+    x, y, vx, vy = x0.copy(), y0.copy(), vx0.copy(), vy0.copy()
+    nsteps = 50
+    for s in range(nsteps):
+        oracle.push_f64(x, y, vx, vy, ex, ey, lx, ly, dt, qm, 0.5 if s == 0 else 1.0)
+    t = nsteps * dt
+    assert np.allclose(x, x0 + vx0 * t + 0.5 * qm * E0x * t * t, rtol=0, atol=1e-9)
+    assert np.allclose(y, y0 + vy0 * t + 0.5 * qm * E0y * t * t, rtol=0, atol=1e-9)
+    # staggered velocity v_{n-1/2}
+    assert np.allclose(vx, vx0 + qm * E0x * (t - 0.5 * dt), atol=1e-12)
+
+
+def test_drift_wraps_periodically(oracle):
+    nx = ny = 8
+    lx, ly = 4.0, 3.0
+    z = np.zeros((nx, ny))
+    x, y = np.array([3.9, 0.05]), np.array([0.1, 2.95])
+    vx, vy = np.array([2.0, -1.0]), np.array([-2.0, 1.0])
+    oracle.push_f64(x, y, vx, vy, z, z, lx, ly, 0.1, -1.0, 1.0)
+    assert np.allclose(x, [0.1, 3.95], atol=1e-14) and np.allclose(y, [2.9, 0.05], atol=1e-14)
+
+
+def test_zero_self_force(oracle):
+    """A lone particle feels no force from its own charge (same CIC weights for deposit and
+    gather, symmetric Green operator, antisymmetric central difference)."""
+    nx, ny, lx, ly = 16, 16, FOUR_PI, FOUR_PI
+    rng = np.random.default_rng(6)
+    for _ in range(5):
+        x, y = rng.uniform(0, lx, 1), rng.uniform(0, ly, 1)
+        ex, ey = oracle.efield(oracle.poisson(oracle.deposit_f64(x, y, nx, ny, lx, ly, -1.0),
+                                              lx, ly), lx, ly)
+        epx, epy = np.zeros(1), np.zeros(1)
+        oracle.or_gather_f64(1, oracle.p(x), oracle.p(y), oracle.p(ex), oracle.p(ey), nx, ny,
+                             lx, ly, oracle.p(epx), oracle.p(epy))
+        scale = np.abs(ex).max()
+        assert abs(epx[0]) < 1e-13 * scale and abs(epy[0]) < 1e-13 * scale
+
+
+def test_f32_mirror_push_tracks_f64(oracle):
+    nx, ny, lx, ly = 16, 16, FOUR_PI, FOUR_PI
+    rng = np.random.default_rng(8)
+    n = 4000
+    x, y = rng.uniform(0, lx, n), rng.uniform(0, ly, n)
+    vx, vy = rng.normal(size=n), rng.normal(size=n)
+    ex, ey = rng.normal(size=(nx, ny)), rng.normal(size=(nx, ny))
+    a = [v.astype(np.float32) for v in (x, y, vx, vy)]
+    b = [v.astype(np.float32).astype(np.float64) for v in (x, y, vx, vy)]
+    oracle.push_f32(*a, ex, ey, lx, ly, 0.1, -1.0, 1.0)
+    oracle.push_f64(*b, ex.astype(np.float32).astype(np.float64),
+                    ey.astype(np.float32).astype(np.float64), lx, ly, 0.1, -1.0, 1.0,
+                    quantized=1)
+    for u, v in zip(a, b):
+        d = np.abs(u - v)
+        d = np.minimum(d, np.abs(d - lx))     # a wrap on either side of L
+        assert d.max() < 1e-5
+
+
+# ------------------------------------------------------------------ own math
+def test_own_sincos_and_log(oracle):
+    s, c = ctypes.c_double(), ctypes.c_double()
+    for u in np.linspace(0, 1, 257)[:-1]:
+        oracle.or_sincos2pi(float(u), ctypes.byref(s), ctypes.byref(c))
+        # math.sin's own argument 2*pi*u carries a rounding of up to ~4e-16
+        assert abs(s.value - math.sin(2 * math.pi * u)) < 1e-15
+        assert abs(c.value - math.cos(2 * math.pi * u)) < 1e-15
+    for v in [1e-300, 1e-10, 0.3, 0.5, 1.0, 2.0 ** -24, 0.9999999]:
+        assert abs(oracle.or_log(v) - math.log(v)) <= 4e-16 * max(1.0, abs(math.log(v)))
+
+
+def test_mcc_cross_sections_closed_form(oracle):
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    for e in [0.5, 5.0, 11.4, 11.6, 15.7, 15.9, 50.0, 300.0]:
+        oracle.or_mcc_cross_sections(e, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        assert a.value == pytest.approx(1e-19 / (1 + e / 10.0), rel=1e-15)
+        assert b.value == (pytest.approx(2e-21 * (1 - 11.5 / e), rel=1e-15) if e > 11.5 else 0)
+        assert c.value == (pytest.approx(3e-20 * math.log(e / 15.8) / e, rel=1e-15)
+                           if e > 15.8 else 0)
+
+
+# ------------------------------------------------------------------ whole step
+def thermal_run(oracle, nsteps, nx=32, ppc=16, vth=1.0, seed=21):
+    lx = ly = FOUR_PI
+    x, y, vx, vy = oracle.init_landau(nx, nx, lx, ly, ppc, vth=vth, alpha=0.0, seed=seed)
+    n = x.size
+    qp = -(lx * ly) / n
+    diag = oracle.run_f64(x, y, vx, vy, nx, nx, lx, ly, 0.1, -1.0, qp, nsteps)
+    return diag, n
+
+
+def test_momentum_conserved(oracle):
+    diag, n = thermal_run(oracle, 100)
+    p0 = math.sqrt(2 * diag[0, 1] * n) / n     # thermal momentum scale per particle
+    assert np.abs(diag[:, 2] - diag[0, 2]).max() < 1e-10 * p0 * n
+    assert np.abs(diag[:, 3] - diag[0, 3]).max() < 1e-10 * p0 * n
+
+
+def test_energy_drift_bounded(oracle):
+    diag, _ = thermal_run(oracle, 200)
+    tot = diag[:, 0] + diag[:, 1]
+    assert np.abs(tot - tot[0]).max() < 0.01 * tot[0]
+
+
+def test_two_stream_growth_rate(oracle):
+    """Config-1 two-stream (64^2, 4 pi, v0 = 1, vth 0.01, mode 1, dt 0.1) at 16 ppc: the
+    |rho_hat(1,0)| growth rate in the linear phase is the cold-beam gamma = 0.3406."""
+    nx = 64
+    lx = ly = FOUR_PI
+    x, y, vx, vy = oracle.init_two_stream(nx, nx, lx, ly, 16, v0=1.0, vth=0.01, amp=1e-3)
+    n = x.size
+    diag = oracle.run_f64(x, y, vx, vy, nx, nx, lx, ly, 0.1, -1.0, -(lx * ly) / n, 200)
+    t = (np.arange(200) + 0) * 0.1
+    sel = (t >= 8) & (t <= 16)
+    g = np.polyfit(t[sel], np.log(diag[sel, 4]), 1)[0]
+    assert abs(g - 0.340625) < 0.03
