@@ -74,7 +74,9 @@ def cpu_baseline(parts, cfg, nsteps):
 
 
 def batch_ratio(steps=300):
-    """48 ragged runs: one batched context vs the same runs one at a time."""
+    """48 ragged runs: one batched context vs the same runs one at a time.  Both sides are
+    autotuned first (the batch over tile/halo/sort cadence for all runs at once, each
+    sequential run on its own), so the ratio compares the two launch structures."""
     import torch
     from pic_amd import Batch, Simulation, presets
     runs, _ = presets.batch_48()
@@ -82,6 +84,8 @@ def batch_ratio(steps=300):
     b = Batch([c for c, _ in runs])
     for r, (_, p) in enumerate(runs):
         b.set_particles(*p, run=r)
+    b.autotune(16)
+    btune = b.tuning()
     b.step(10)
     b.sync()
     t0 = time.perf_counter()
@@ -93,6 +97,7 @@ def batch_ratio(steps=300):
     for c, p in runs:
         s = Simulation(c)
         s.set_particles(*p)
+        s.autotune(8)
         s.step(10)
         s.sync()
         sims.append(s)
@@ -106,7 +111,8 @@ def batch_ratio(steps=300):
         s.close()
     return {"runs": len(runs), "particles": total, "steps": steps,
             "batched_particle_steps_per_s": total * steps / tb,
-            "sequential_particle_steps_per_s": total * steps / ts, "ratio": ts / tb}
+            "sequential_particle_steps_per_s": total * steps / ts, "ratio": ts / tb,
+            "batch_tuning": btune}
 
 
 def main():
@@ -190,8 +196,10 @@ def main():
             line["batched_vs_sequential"] = {"error": repr(e)}
     if not args.no_adjoint:
         try:
+            from pic_amd import _lib
             from pic_amd.adjoint import time_adjoint
-            line["adjoint_over_forward"] = time_adjoint()
+            line["adjoint_over_forward"] = {"f32": time_adjoint(precision=_lib.PIC_FP32),
+                                            "f64": time_adjoint(precision=_lib.PIC_FP64)}
         except Exception as e:
             line["adjoint_over_forward"] = {"error": repr(e)}
     print(json.dumps(line))

@@ -39,11 +39,14 @@ struct pic_adjoint {
   int nrp = 0, ncl = 0, nrw = 0;
   double* part = nullptr;
   bool forward_done = false;
+  hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // captured step loops (made on first use)
   std::vector<void*> all;
 
   ~pic_adjoint() {
     (void)hipSetDevice(device);
     if (s) (void)hipStreamSynchronize(s);
+    if (gfwd) (void)hipGraphExecDestroy(gfwd);
+    if (gbwd) (void)hipGraphExecDestroy(gbwd);
     for (void* p : all) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -64,77 +67,108 @@ struct pic_adjoint {
 
 namespace {
 
-template <typename R>
-int field(pic_adjoint& a) {
+constexpr int kEnergyBlocks = 256;
+
+// charge of the fixed-point sums (which it clears) -> E at `E` (padded, particle precision)
+void field(pic_adjoint& a, void* E) {
   const long long G = (long long)a.g.nx * a.g.ny;
-  PIC_HIP(hipMemsetAsync(a.hi, 0, sizeof(unsigned long long) * G, a.s));
-  if (a.fp64) PIC_HIP(hipMemsetAsync(a.lo, 0, sizeof(unsigned long long) * G, a.s));
-  launch_op_deposit<R>(a.s, (const R*)a.x, (const R*)a.y, a.n, a.g.nx, a.g.ny, a.g.lx, a.g.ly,
-                       a.hi, a.lo);
   const double scale = a.qp / (a.g.dx * a.g.dy) * (a.fp64 ? 1.0 : std::ldexp(1.0, -36));
-  launch_op_fixed_to_double(a.s, a.hi, a.lo, G, a.fp64, scale, a.rho);
+  adj_rho(a.s, G, a.fp64, scale, a.hi, a.lo, a.rho);
   FieldArgs f;
   std::memset(&f, 0, sizeof(f));
   f.runs = a.dd; f.st = a.st;
-  f.S = a.S; f.Sx = a.Sx; f.Sy = a.Sy; f.E = a.E; f.phi = nullptr; f.tw = a.tw;
+  f.S = a.S; f.Sx = a.Sx; f.Sy = a.Sy; f.E = E; f.phi = nullptr; f.tw = a.tw;
   f.row_pair_items = a.rp; f.n_row_pair_items = a.nrp;
   f.col_items = a.cl; f.n_col_items = a.ncl;
   f.row_items = a.rw; f.n_row_items = a.nrw;
   f.nruns = 1; f.fp64 = a.fp64; f.want_phi = 0; f.max_nx = a.g.nx; f.max_ny = a.g.ny;
   f.rho_in = a.rho;
   launch_field(a.s, f, nullptr);
-  return PIC_OK;
 }
 
+inline double kick(int k) { return k == 0 ? 0.5 : 1.0; }
+
+// Forward steps 0..N-1 from the state (x, y, vx, vy) whose charge is already in hi/lo,
+// saving (x, y, vx, vy) every K steps; then E_N and its energy partials.
 template <typename R>
-int run_steps(pic_adjoint& a, int k0, int k1, bool save_ckpt, bool save_traj) {
+void record_forward(pic_adjoint& a) {
+  R *x = (R*)a.x, *y = (R*)a.y;
+  for (int k = 0; k < a.nsteps; ++k) {
+    field(a, a.E);
+    R* ck = (k % a.K == 0) ? (R*)a.at(a.ckpt, (size_t)(k / a.K) * 4 * a.n) : nullptr;
+    adj_push<R>(a.s, a.g, x, y, x, y, (R*)a.vx, (R*)a.vy, (const R*)a.E, a.n, kick(k), ck, a.hi,
+                a.lo);
+  }
+  field(a, a.E);
+  adj_energy<R>(a.s, a.g, (const R*)a.E, a.part, kEnergyBlocks);
+}
+
+// Reverse sweep for dL = 1 (the caller scales the outputs): seed from E_N, then for each
+// segment restore its checkpoint, recompute its steps into the trajectory buffers and run
+// the reverse steps over them.
+template <typename R>
+void record_backward(pic_adjoint& a) {
   const size_t n = (size_t)a.n;
-  for (int k = k0; k < k1; ++k) {
-    if (save_ckpt && k % a.K == 0) {
-      char* c = a.at(a.ckpt, (size_t)(k / a.K) * 4 * n);
-      PIC_HIP(hipMemcpyAsync(c, a.x, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-      PIC_HIP(hipMemcpyAsync(c + a.rs * n, a.y, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-      PIC_HIP(hipMemcpyAsync(c + 2 * a.rs * n, a.vx, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-      PIC_HIP(hipMemcpyAsync(c + 3 * a.rs * n, a.vy, a.rs * n, hipMemcpyDeviceToDevice, a.s));
+  adj_seed<R>(a.s, a.g, (const R*)a.E, 2.0 * a.g.dx * a.g.dy, a.lE);
+  adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
+  for (double* p : {a.lx, a.ly, a.lvx, a.lvy}) adj_zero(a.s, p, a.n);
+  const double cq = a.qp / (a.g.dx * a.g.dy);
+  auto X = [&](int slot) { return (R*)a.at(a.trx, (size_t)slot * n); };
+  auto Y = [&](int slot) { return (R*)a.at(a.try_, (size_t)slot * n); };
+  auto Ek = [&](int slot) { return (R*)a.at(a.trE, (size_t)slot * a.epad()); };
+  for (int seg = a.nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
+    adj_restore<R>(a.s, a.g, (const R*)a.at(a.ckpt, (size_t)seg * 4 * n), a.n, X(0), Y(0),
+                   (R*)a.vx, (R*)a.vy, a.hi, a.lo);
+    for (int k = k0; k < k1; ++k) {
+      const int sl = k - k0;
+      field(a, Ek(sl));
+      const bool last = (k + 1 == k1);  // x_{k1} is not needed as charge here
+      adj_push<R>(a.s, a.g, X(sl), Y(sl), X(sl + 1), Y(sl + 1), (R*)a.vx, (R*)a.vy, Ek(sl), a.n,
+                  kick(k), nullptr, last ? nullptr : a.hi, a.lo);
     }
-    if (save_traj) {
-      PIC_HIP(hipMemcpyAsync(a.at(a.trx, (size_t)(k - k0) * n), a.x, a.rs * n,
-                             hipMemcpyDeviceToDevice, a.s));
-      PIC_HIP(hipMemcpyAsync(a.at(a.try_, (size_t)(k - k0) * n), a.y, a.rs * n,
-                             hipMemcpyDeviceToDevice, a.s));
+    for (int k = k1; k > k0; --k) {
+      const int sl = k - k0;
+      adj_pass<R>(a.s, a.g, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1), Ek(sl - 1),
+                  kick(k - 1) * a.g.dt * a.g.qm, a.lx, a.ly, a.lvx, a.lvy, a.lE, a.n);
+      adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
     }
-    int rc = field<R>(a);
-    if (rc) return rc;
-    if (save_traj)
-      PIC_HIP(hipMemcpyAsync(a.at(a.trE, (size_t)(k - k0) * a.epad()), a.E, a.rs * a.epad(),
-                             hipMemcpyDeviceToDevice, a.s));
-    adj_push<R>(a.s, a.g, (R*)a.x, (R*)a.y, (R*)a.vx, (R*)a.vy, (const R*)a.E, a.n,
-                k == 0 ? 0.5 : 1.0);
   }
-  if (save_traj) {
-    PIC_HIP(hipMemcpyAsync(a.at(a.trx, (size_t)(k1 - k0) * n), a.x, a.rs * n,
-                           hipMemcpyDeviceToDevice, a.s));
-    PIC_HIP(hipMemcpyAsync(a.at(a.try_, (size_t)(k1 - k0) * n), a.y, a.rs * n,
-                           hipMemcpyDeviceToDevice, a.s));
-  }
-  PIC_HIP(hipGetLastError());
+  // deposit Jacobian of step 0 at x_0 (slot 0 of segment 0)
+  adj_pass<R>(a.s, a.g, X(0), Y(0), a.lrho, cq, nullptr, nullptr, nullptr, 0.0, a.lx, a.ly,
+              a.lvx, a.lvy, a.lE, a.n);
+}
+
+template <typename F>
+int capture(pic_adjoint& a, hipGraphExec_t* out, F&& body) {
+  hipGraph_t g = nullptr;
+  PIC_HIP(hipStreamBeginCapture(a.s, hipStreamCaptureModeThreadLocal));
+  body();
+  hipError_t e = hipStreamEndCapture(a.s, &g);
+  if (e != hipSuccess)
+    return fail(PIC_ERR_HIP, std::string("adjoint stream capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return fail(PIC_ERR_HIP, std::string("adjoint graph instantiate: ") + hipGetErrorString(e));
   return PIC_OK;
 }
 
 template <typename R>
 int forward_t(pic_adjoint& a, const double* amps, const void* vx0, const void* vy0,
               double* loss) {
-  adj_init<R>(a.s, a.xl, a.yl, a.n, amps, a.mkx, a.mky, a.nm, a.g.lx, (R*)a.x, (R*)a.y);
+  if (!a.gfwd) {
+    int rc = capture(a, &a.gfwd, [&] { record_forward<R>(a); });
+    if (rc) return rc;
+  }
+  adj_init<R>(a.s, a.g, a.xl, a.yl, a.n, amps, a.mkx, a.mky, a.nm, (R*)a.x, (R*)a.y, a.hi,
+              a.lo);
   PIC_HIP(hipMemcpyAsync(a.vx, vx0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
   PIC_HIP(hipMemcpyAsync(a.vy, vy0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
-  int rc = run_steps<R>(a, 0, a.nsteps, true, false);
-  if (rc) return rc;
-  rc = field<R>(a);
-  if (rc) return rc;
-  const int nb = 256;
-  adj_energy<R>(a.s, a.g, (const R*)a.E, a.part, nb);
-  std::vector<double> h(nb);
-  PIC_HIP(hipMemcpyAsync(h.data(), a.part, sizeof(double) * nb, hipMemcpyDeviceToHost, a.s));
+  PIC_HIP(hipGraphLaunch(a.gfwd, a.s));
+  std::vector<double> h(kEnergyBlocks);
+  PIC_HIP(hipMemcpyAsync(h.data(), a.part, sizeof(double) * kEnergyBlocks,
+                         hipMemcpyDeviceToHost, a.s));
   PIC_HIP(hipStreamSynchronize(a.s));
   double sum = 0.0;
   for (double v : h) sum += v;
@@ -145,43 +179,16 @@ int forward_t(pic_adjoint& a, const double* amps, const void* vx0, const void* v
 
 template <typename R>
 int backward_t(pic_adjoint& a, double dloss, double* grad_amps, void* gvx, void* gvy) {
-  const size_t n = (size_t)a.n;
-  const long long G = (long long)a.g.nx * a.g.ny;
-  // x, y hold x_N after the forward: seed lE = 2 dL E_N dx dy, lrho_N = M^T lE
-  int rc = field<R>(a);
-  if (rc) return rc;
-  adj_seed<R>(a.s, a.g, (const R*)a.E, 2.0 * dloss * a.g.dx * a.g.dy, a.lE);
-  adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
-  for (double* p : {a.lx, a.ly, a.lvx, a.lvy})
-    PIC_HIP(hipMemsetAsync(p, 0, sizeof(double) * n, a.s));
-  const double cq = a.qp / (a.g.dx * a.g.dy);
-  for (int seg = a.nseg - 1; seg >= 0; --seg) {
-    const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
-    const char* c = a.at(a.ckpt, (size_t)seg * 4 * n);
-    PIC_HIP(hipMemcpyAsync(a.x, c, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-    PIC_HIP(hipMemcpyAsync(a.y, c + a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-    PIC_HIP(hipMemcpyAsync(a.vx, c + 2 * a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-    PIC_HIP(hipMemcpyAsync(a.vy, c + 3 * a.rs * n, a.rs * n, hipMemcpyDeviceToDevice, a.s));
-    if ((rc = run_steps<R>(a, k0, k1, false, true))) return rc;
-    for (int k = k1; k > k0; --k) {
-      PIC_HIP(hipMemsetAsync(a.lE, 0, sizeof(double) * 2 * G, a.s));
-      const double ck1 = (k - 1 == 0 ? 0.5 : 1.0) * a.g.dt * a.g.qm;
-      adj_pass<R>(a.s, a.g, (const R*)a.at(a.trx, (size_t)(k - k0) * n),
-                  (const R*)a.at(a.try_, (size_t)(k - k0) * n), a.lrho, cq,
-                  (const R*)a.at(a.trx, (size_t)(k - 1 - k0) * n),
-                  (const R*)a.at(a.try_, (size_t)(k - 1 - k0) * n),
-                  (const R*)a.at(a.trE, (size_t)(k - 1 - k0) * a.epad()), ck1, a.lx, a.ly,
-                  a.lvx, a.lvy, a.lE, a.n);
-      adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
-    }
+  if (!a.gbwd) {
+    int rc = capture(a, &a.gbwd, [&] { record_backward<R>(a); });
+    if (rc) return rc;
   }
-  // finish step 0 (deposit Jacobian at x_0); trajectory slot 0 of segment 0 holds x_0
-  adj_pass<R>(a.s, a.g, (const R*)a.trx, (const R*)a.try_, a.lrho, cq, nullptr, nullptr, nullptr,
-              0.0, a.lx, a.ly, a.lvx, a.lvy, a.lE, a.n);
+  PIC_HIP(hipGraphLaunch(a.gbwd, a.s));
   if (grad_amps)
-    adj_amp_grad(a.s, a.lx, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, 256, 1.0, grad_amps);
-  if (gvx) adj_to_r<R>(a.s, a.lvx, 1.0, (R*)gvx, a.n);
-  if (gvy) adj_to_r<R>(a.s, a.lvy, 1.0, (R*)gvy, a.n);
+    adj_amp_grad(a.s, a.lx, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, kEnergyBlocks, dloss,
+                 grad_amps);
+  if (gvx) adj_to_r<R>(a.s, a.lvx, dloss, (R*)gvx, a.n);
+  if (gvy) adj_to_r<R>(a.s, a.lvy, dloss, (R*)gvy, a.n);
   PIC_HIP(hipGetLastError());
   PIC_HIP(hipStreamSynchronize(a.s));
   return PIC_OK;
@@ -257,7 +264,7 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   PIC_HIP(A.get(&A.tw, PIC_FFT_MAX / 2));
   PIC_HIP(A.get(&A.dd, 1));
   PIC_HIP(A.get(&A.st, 1));
-  PIC_HIP(A.get(&A.part, 256 * std::max(nmodes, 1)));
+  PIC_HIP(A.get(&A.part, kEnergyBlocks * std::max(nmodes, 1)));
   PIC_HIP(hipMemcpy(A.xl, xl, sizeof(double) * n, hipMemcpyHostToDevice));
   PIC_HIP(hipMemcpy(A.yl, yl, sizeof(double) * n, hipMemcpyHostToDevice));
   std::vector<double> kx(std::max(nmodes, 1), 0.0), ky(std::max(nmodes, 1), 0.0);
@@ -281,6 +288,8 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   d.row_item_begin = 0; d.row_item_end = g.nx;
   PIC_HIP(hipMemcpy(A.dd, &d, sizeof(d), hipMemcpyHostToDevice));
   PIC_HIP(hipMemset(A.st, 0, sizeof(RunState)));
+  PIC_HIP(hipMemset(A.hi, 0, sizeof(unsigned long long) * G));
+  PIC_HIP(hipMemset(A.lo, 0, sizeof(unsigned long long) * G));
   std::vector<int2> rp, cl, rw;
   for (int i = 0; i < g.nx; i += 2) rp.push_back(make_int2(0, i));
   for (int k = 0; k <= g.ny / 2; ++k) cl.push_back(make_int2(0, k));

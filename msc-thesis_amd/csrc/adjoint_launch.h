@@ -9,11 +9,19 @@ struct AdjGrid {
 };
 
 template <typename R>
-void adj_init(hipStream_t s, const double* xl, const double* yl, int64_t n, const double* amps,
-              const double* mkx, const double* mky, int nm, double lx, R* x, R* y);
+void adj_init(hipStream_t s, const AdjGrid& g, const double* xl, const double* yl, int64_t n,
+              const double* amps, const double* mkx, const double* mky, int nm, R* x, R* y,
+              unsigned long long* hi, unsigned long long* lo);
 template <typename R>
-void adj_push(hipStream_t s, const AdjGrid& g, R* x, R* y, R* vx, R* vy, const R* E, int64_t n,
-              double c);
+void adj_push(hipStream_t s, const AdjGrid& g, const R* xi, const R* yi, R* xo, R* yo, R* vx,
+              R* vy, const R* E, int64_t n, double c, R* ck, unsigned long long* hi,
+              unsigned long long* lo);
+template <typename R>
+void adj_restore(hipStream_t s, const AdjGrid& g, const R* ck, int64_t n, R* x, R* y, R* vx,
+                 R* vy, unsigned long long* hi, unsigned long long* lo);
+void adj_rho(hipStream_t s, long long G, int fp64, double scale, unsigned long long* hi,
+             unsigned long long* lo, double* rho);
+void adj_zero(hipStream_t s, double* a, long long n);
 template <typename R>
 void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const double* lrho,
               double cq, const R* xp, const R* yp, const R* Eprev, double ck1, double* lx,
@@ -22,7 +30,7 @@ template <typename R>
 void adj_seed(hipStream_t s, const AdjGrid& g, const R* E, double sc, double* lE);
 template <typename R>
 void adj_energy(hipStream_t s, const AdjGrid& g, const R* E, double* part, int nblocks);
-void adj_mt(hipStream_t s, const AdjGrid& g, const double* lE, double2* A, double* lrho,
+void adj_mt(hipStream_t s, const AdjGrid& g, double* lE, double2* A, double* lrho,
             const double2* tw);
 void adj_amp_grad(hipStream_t s, const double* lx, const double* xl, const double* yl, int64_t n,
                   const double* mkx, const double* mky, int nm, double* part, int nblocks,

@@ -261,7 +261,7 @@ int Ctx::alloc() {
   PIC_HIP(dalloc(&sb.big_list, 2 * ctotal));
   PIC_HIP(dalloc(&sb.big_count, 1));
   PIC_HIP(dalloc(&sb.scratch, 2 * ptotal));
-  PIC_HIP(dalloc(&tickets, 2));
+  PIC_HIP(dalloc(&tickets, 2 * nruns));  // per-run tickets: push [0, nruns), field [nruns, 2 nruns)
   PIC_HIP(dalloc(&diag, (size_t)nruns * PIC_DIAG_CAP * 8));
   PIC_HIP(dalloc(&d_coll_counts, 4 * nruns));
   PIC_HIP(dalloc(&flag, 1));
@@ -284,7 +284,7 @@ int Ctx::alloc() {
   PIC_HIP(hipMemset(d_st, 0, sizeof(RunState) * nruns));
   PIC_HIP(hipMemset(rho, 0, sizeof(unsigned long long) * 2 * gsum));
   if (fp64) PIC_HIP(hipMemset(rho_lo, 0, sizeof(unsigned long long) * 2 * gsum));
-  PIC_HIP(hipMemset(tickets, 0, sizeof(unsigned int) * 2));
+  PIC_HIP(hipMemset(tickets, 0, sizeof(unsigned int) * 2 * nruns));
   PIC_HIP(hipMemset(diag, 0, sizeof(double) * (size_t)nruns * PIC_DIAG_CAP * 8));
   PIC_HIP(hipMemset(d_coll_counts, 0, sizeof(long long) * 4 * nruns));
   PIC_HIP(hipMemset(E, 0, rs * 2 * esum));
@@ -306,7 +306,7 @@ FieldArgs Ctx::field_args(int want_phi) const {
   a.row_pair_items = d_rowpair; a.n_row_pair_items = (int)rowpair.size();
   a.col_items = d_cols; a.n_col_items = (int)cols.size();
   a.row_items = d_rows; a.n_row_items = (int)rows.size();
-  a.partials = partials_rows; a.ticket = tickets + 1; a.diag = diag;
+  a.partials = partials_rows; a.ticket = tickets + nruns; a.diag = diag;
   a.nruns = nruns; a.fp64 = fp64; a.want_phi = want_phi;
   a.max_nx = max_nx; a.max_ny = max_ny;
   a.rho_in = nullptr;

@@ -15,6 +15,7 @@
 // fractions (the fixed-point quantisation of the forward weights is below 2^-18 of a cell).
 #include <type_traits>
 #include "fft_lds.h"
+#include "deposit.h"
 #include "launch.h"
 #include "adjoint_launch.h"
 
@@ -24,47 +25,68 @@ template <typename R>
 using R2T = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
 
 template <typename R>
-__device__ __forceinline__ R wrapp(R x, R L) { return wrap_pos(x, L); }
+__device__ __forceinline__ Geo<R> geo_of(const AdjGrid& g) {
+  return make_geo<R>(g.nx, g.ny, g.lx, g.ly);
+}
 
+// x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl) (wrapped), y0 = yl; deposits x0's charge
 template <typename R>
-__global__ __launch_bounds__(PIC_WG) void k_adj_init(const double* __restrict__ xl,
+__global__ __launch_bounds__(PIC_WG) void k_adj_init(AdjGrid g, const double* __restrict__ xl,
                                                      const double* __restrict__ yl, int64_t n,
                                                      const double* __restrict__ amps,
                                                      const double* __restrict__ mkx,
                                                      const double* __restrict__ mky, int nm,
-                                                     double lx, R* __restrict__ x,
-                                                     R* __restrict__ y) {
+                                                     R* __restrict__ x, R* __restrict__ y,
+                                                     unsigned long long* hi,
+                                                     unsigned long long* lo) {
+  const Geo<R> q = geo_of<R>(g);
   for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * PIC_WG) {
     const double a = xl[p], b = yl[p];
     double d = 0.0;
     for (int m = 0; m < nm; ++m) d += amps[m] * cos(mkx[m] * a + mky[m] * b);
-    double x0 = a + d;
-    x0 = wrap_pos(x0, lx);
-    x[p] = wrapp((R)x0, (R)lx);
-    y[p] = (R)b;
+    const R xr = wrap_pos((R)wrap_pos(a + d, g.lx), (R)g.lx);
+    const R yr = (R)b;
+    x[p] = xr;
+    y[p] = yr;
+    int cx, cy;
+    long long w[4];
+    deposit_weights<R>(xr, yr, q, cx, cy, w);
+    deposit_global<R>(hi, lo, cx, cy, w, q);
   }
 }
 
-// forward push of particle-indexed arrays (the engine's step without tiling)
+// One forward step of particle-indexed arrays: gather with the fixed-point weights, kick
+// (factor c), drift from (xi, yi) to (xo, yo) (may alias), then deposit the new position's
+// charge into hi/lo (skipped when hi is NULL).  ck != NULL receives the state before the
+// step (checkpoint layout x | y | vx | vy, n each).
 template <typename R>
-__global__ __launch_bounds__(PIC_WG) void k_adj_push(AdjGrid g, R* __restrict__ x,
-                                                     R* __restrict__ y, R* __restrict__ vx,
+__global__ __launch_bounds__(PIC_WG) void k_adj_push(AdjGrid g, const R* xi, const R* yi,
+                                                     R* xo, R* yo, R* __restrict__ vx,
                                                      R* __restrict__ vy,
                                                      const R* __restrict__ Epad, int64_t n,
-                                                     double c) {
-  const R dx = (R)g.dx, dy = (R)g.dy, Lx = (R)g.lx, Ly = (R)g.ly;
+                                                     double c, R* __restrict__ ck,
+                                                     unsigned long long* hi,
+                                                     unsigned long long* lo) {
+  const Geo<R> q = geo_of<R>(g);
+  const R Lx = (R)g.lx, Ly = (R)g.ly;
   const R hk = (R)(c * g.dt * g.qm), dtr = (R)g.dt;
   const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
   const R2T<R>* E = reinterpret_cast<const R2T<R>*>(Epad);
   const int ny1 = g.ny + 1;
   for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * PIC_WG) {
-    R xr = x[p], yr = y[p], vxr = vx[p], vyr = vy[p];
+    R xr = xi[p], yr = yi[p], vxr = vx[p], vyr = vy[p];
+    if (ck) {
+      ck[p] = xr;
+      ck[n + p] = yr;
+      ck[2 * n + p] = vxr;
+      ck[3 * n + p] = vyr;
+    }
     int cx, cy;
     R fx, fy;
-    locate(xr, dx, g.nx, cx, fx);
-    locate(yr, dy, g.ny, cy, fy);
+    locate_w(xr, q.idx, g.nx, cx, fx);
+    locate_w(yr, q.idy, g.ny, cy, fy);
     const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
     const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
     const long long i00 = (long long)cx * ny1 + cy;
@@ -79,11 +101,67 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_push(AdjGrid g, R* __restrict__ 
     epy = epy + (gx1 * gy1) * e11.y;
     vxr = vxr + hk * epx;
     vyr = vyr + hk * epy;
-    x[p] = wrap_pos((R)(xr + dtr * vxr), Lx);
-    y[p] = wrap_pos((R)(yr + dtr * vyr), Ly);
+    xr = wrap_pos((R)(xr + dtr * vxr), Lx);
+    yr = wrap_pos((R)(yr + dtr * vyr), Ly);
+    xo[p] = xr;
+    yo[p] = yr;
     vx[p] = vxr;
     vy[p] = vyr;
+    if (hi) {
+      long long w[4];
+      deposit_weights<R>(xr, yr, q, cx, cy, w);
+      deposit_global<R>(hi, lo, cx, cy, w, q);
+    }
   }
+}
+
+// restore a checkpoint (x | y | vx | vy) into (x, y) and (vx, vy); deposit x's charge
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_adj_restore(AdjGrid g, const R* __restrict__ ck,
+                                                        int64_t n, R* __restrict__ x,
+                                                        R* __restrict__ y, R* __restrict__ vx,
+                                                        R* __restrict__ vy,
+                                                        unsigned long long* hi,
+                                                        unsigned long long* lo) {
+  const Geo<R> q = geo_of<R>(g);
+  for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * PIC_WG) {
+    const R xr = ck[p], yr = ck[n + p];
+    x[p] = xr;
+    y[p] = yr;
+    vx[p] = ck[2 * n + p];
+    vy[p] = ck[3 * n + p];
+    int cx, cy;
+    long long w[4];
+    deposit_weights<R>(xr, yr, q, cx, cy, w);
+    deposit_global<R>(hi, lo, cx, cy, w, q);
+  }
+}
+
+// charge density from the fixed-point sums (scale = qp/dA, times 2^-36 for FP32 words);
+// the sums are zeroed for the next deposit
+__global__ __launch_bounds__(PIC_WG) void k_adj_rho(long long G, int fp64, double scale,
+                                                    unsigned long long* hi,
+                                                    unsigned long long* lo,
+                                                    double* __restrict__ rho) {
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
+       q += (long long)gridDim.x * PIC_WG) {
+    const long long h = (long long)hi[q];
+    hi[q] = 0ull;
+    if (fp64) {
+      const long long l = (long long)lo[q];
+      lo[q] = 0ull;
+      rho[q] = ((double)h * 9.313225746154785e-10 + (double)l * 8.673617379884035e-19) * scale;
+    } else {
+      rho[q] = (double)h * scale;
+    }
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_adj_zero(double* __restrict__ a, long long n) {
+  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < n;
+       q += (long long)gridDim.x * PIC_WG)
+    a[q] = 0.0;
 }
 
 // continuous CIC weights and their gradients at x (double arithmetic)
@@ -203,13 +281,17 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_energy(AdjGrid g, const R* __res
 
 // ---- M^T by a complex 2D FFT of the packed pair z = lEx + i lEy
 // rows: z[i][j] -> A[k][i] = FFT_y(z[i])[k]  (transposed, all k)
-__global__ __launch_bounds__(PIC_WG) void k_adj_rows_fwd(AdjGrid g, const double* __restrict__ lE,
+// (lE is zeroed as it is read, ready for the next reverse deposit)
+__global__ __launch_bounds__(PIC_WG) void k_adj_rows_fwd(AdjGrid g, double* __restrict__ lE,
                                                          double2* __restrict__ A,
                                                          const double2* __restrict__ tw) {
   extern __shared__ double2 s[];
   const int i = blockIdx.x, ny = g.ny, G = g.nx * g.ny;
-  for (int j = threadIdx.x; j < ny; j += PIC_WG)
+  for (int j = threadIdx.x; j < ny; j += PIC_WG) {
     s[bitrev(j, g.logny)] = make_double2(lE[i * ny + j], lE[G + i * ny + j]);
+    lE[i * ny + j] = 0.0;
+    lE[G + i * ny + j] = 0.0;
+  }
   __syncthreads();
   fft_lds(s, ny, g.logny, -1, tw);
   for (int k = threadIdx.x; k < ny; k += PIC_WG) A[(long long)k * g.nx + i] = s[k];
@@ -323,14 +405,28 @@ inline int gridn(long long n) {
 }  // namespace
 
 template <typename R>
-void adj_init(hipStream_t s, const double* xl, const double* yl, int64_t n, const double* amps,
-              const double* mkx, const double* mky, int nm, double lx, R* x, R* y) {
-  k_adj_init<R><<<gridn(n), PIC_WG, 0, s>>>(xl, yl, n, amps, mkx, mky, nm, lx, x, y);
+void adj_init(hipStream_t s, const AdjGrid& g, const double* xl, const double* yl, int64_t n,
+              const double* amps, const double* mkx, const double* mky, int nm, R* x, R* y,
+              unsigned long long* hi, unsigned long long* lo) {
+  k_adj_init<R><<<gridn(n), PIC_WG, 0, s>>>(g, xl, yl, n, amps, mkx, mky, nm, x, y, hi, lo);
 }
 template <typename R>
-void adj_push(hipStream_t s, const AdjGrid& g, R* x, R* y, R* vx, R* vy, const R* E, int64_t n,
-              double c) {
-  k_adj_push<R><<<gridn(n), PIC_WG, 0, s>>>(g, x, y, vx, vy, E, n, c);
+void adj_push(hipStream_t s, const AdjGrid& g, const R* xi, const R* yi, R* xo, R* yo, R* vx,
+              R* vy, const R* E, int64_t n, double c, R* ck, unsigned long long* hi,
+              unsigned long long* lo) {
+  k_adj_push<R><<<gridn(n), PIC_WG, 0, s>>>(g, xi, yi, xo, yo, vx, vy, E, n, c, ck, hi, lo);
+}
+template <typename R>
+void adj_restore(hipStream_t s, const AdjGrid& g, const R* ck, int64_t n, R* x, R* y, R* vx,
+                 R* vy, unsigned long long* hi, unsigned long long* lo) {
+  k_adj_restore<R><<<gridn(n), PIC_WG, 0, s>>>(g, ck, n, x, y, vx, vy, hi, lo);
+}
+void adj_rho(hipStream_t s, long long G, int fp64, double scale, unsigned long long* hi,
+             unsigned long long* lo, double* rho) {
+  k_adj_rho<<<gridn(G), PIC_WG, 0, s>>>(G, fp64, scale, hi, lo, rho);
+}
+void adj_zero(hipStream_t s, double* a, long long n) {
+  k_adj_zero<<<gridn(n), PIC_WG, 0, s>>>(a, n);
 }
 template <typename R>
 void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const double* lrho,
@@ -347,7 +443,7 @@ template <typename R>
 void adj_energy(hipStream_t s, const AdjGrid& g, const R* E, double* part, int nblocks) {
   k_adj_energy<R><<<nblocks, PIC_WG, 0, s>>>(g, E, part);
 }
-void adj_mt(hipStream_t s, const AdjGrid& g, const double* lE, double2* A, double* lrho,
+void adj_mt(hipStream_t s, const AdjGrid& g, double* lE, double2* A, double* lrho,
             const double2* tw) {
   k_adj_rows_fwd<<<g.nx, PIC_WG, sizeof(double2) * g.ny, s>>>(g, lE, A, tw);
   k_adj_cols<<<g.ny / 2 + 1, PIC_WG, sizeof(double2) * 4 * g.nx, s>>>(g, A, tw);
@@ -366,10 +462,14 @@ void adj_to_r(hipStream_t s, const double* a, double sc, R* out, int64_t n) {
 }
 
 #define ADJ_INST(R)                                                                           \
-  template void adj_init<R>(hipStream_t, const double*, const double*, int64_t, const double*,\
-                            const double*, const double*, int, double, R*, R*);              \
-  template void adj_push<R>(hipStream_t, const AdjGrid&, R*, R*, R*, R*, const R*, int64_t,   \
-                            double);                                                         \
+  template void adj_init<R>(hipStream_t, const AdjGrid&, const double*, const double*,       \
+                            int64_t, const double*, const double*, const double*, int, R*,   \
+                            R*, unsigned long long*, unsigned long long*);                   \
+  template void adj_push<R>(hipStream_t, const AdjGrid&, const R*, const R*, R*, R*, R*, R*,  \
+                            const R*, int64_t, double, R*, unsigned long long*,              \
+                            unsigned long long*);                                            \
+  template void adj_restore<R>(hipStream_t, const AdjGrid&, const R*, int64_t, R*, R*, R*,    \
+                               R*, unsigned long long*, unsigned long long*);                \
   template void adj_pass<R>(hipStream_t, const AdjGrid&, const R*, const R*, const double*,   \
                             double, const R*, const R*, const R*, double, double*, double*,  \
                             double*, double*, double*, int64_t);                             \

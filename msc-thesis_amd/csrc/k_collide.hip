@@ -121,7 +121,7 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   const int take = min(nb, room);
   const int par = (int)(a.st->step & 1);
   unsigned long long* rh = rho + par * gsum + rd.gbase;
-  const float dxf = (float)rd.dx, dyf = (float)rd.dy;
+  const float idx = (float)(1.0 / rd.dx), idy = (float)(1.0 / rd.dy);
   for (int t = threadIdx.x; t < take; t += PIC_WG) {
     const int src = (int)(k[t] & 0xffffffffull);
     const long long d = a.pbase + np + t;
@@ -133,8 +133,8 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
     a.id[d] = (int)(nid + t);
     int cx, cy;
     float fx, fy;
-    locate(bx, dxf, rd.nx, cx, fx);
-    locate(by, dyf, rd.ny, cy, fy);
+    locate_w(bx, idx, rd.nx, cx, fx);
+    locate_w(by, idy, rd.ny, cy, fy);
     const long long one = 1ll << 18;
     const long long ix1 = qweight(fx), iy1 = qweight(fy), ix0 = one - ix1, iy0 = one - iy1;
     const int cxp = (cx + 1 == rd.nx) ? 0 : cx + 1, cyp = (cy + 1 == rd.ny) ? 0 : cy + 1;

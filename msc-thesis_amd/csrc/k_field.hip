@@ -172,22 +172,20 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   if (threadIdx.x == 0) {
     a.partials[blockIdx.x] = s;
     __threadfence();
-    ticket_s = atomicAdd(a.ticket, 1u);
+    ticket_s = atomicAdd(a.ticket + it.x, 1u);
   }
   __syncthreads();
-  if (ticket_s != gridDim.x - 1) return;
+  // the last row of this run sums the run's row partials in row order
+  if (ticket_s != (unsigned int)(rd.row_item_end - rd.row_item_begin - 1)) return;
   __threadfence();
-  for (int r = 0; r < a.nruns; ++r) {
-    const RunDesc d = a.runs[r];
-    double v = 0.0;
-    for (int q = d.row_item_begin + threadIdx.x; q < d.row_item_end; q += PIC_WG) v += a.partials[q];
-    v = block_sum(v, red);
-    if (threadIdx.x == 0) {
-      const long long stp = a.st[r].step;
-      a.diag[(d.dbase + (stp % PIC_DIAG_CAP)) * 8 + 0] = 0.5 * v * d.dx * d.dy;
-    }
+  double v = 0.0;
+  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) v += a.partials[q];
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) {
+    const long long stp = a.st[it.x].step;
+    a.diag[(rd.dbase + (stp % PIC_DIAG_CAP)) * 8 + 0] = 0.5 * v * rd.dx * rd.dy;
+    a.ticket[it.x] = 0u;
   }
-  if (threadIdx.x == 0) *a.ticket = 0u;
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_op_efield(const double* __restrict__ phi,

@@ -10,6 +10,7 @@
 #include <climits>
 #include <type_traits>
 #include "launch.h"
+#include "deposit.h"
 
 namespace {
 
@@ -534,45 +535,6 @@ __global__ void k_fill_u64(unsigned long long* p, long long n) {
 // is accumulated in an LDS window of (T+2H+1)^2 nodes around the tile with int64 atomics
 // and flushed once per workgroup; particles that drifted outside the window since the
 // last sort deposit straight to HBM.  Gather reads the padded E array (L2-resident).
-template <typename R>
-struct Geo {
-  int nx, ny;
-  R dx, dy, Lx, Ly;
-};
-
-template <typename R>
-__device__ __forceinline__ void deposit_weights(R xr, R yr, const Geo<R>& g, int& cx, int& cy,
-                                                long long w[4]) {
-  R fx, fy;
-  locate(xr, g.dx, g.nx, cx, fx);
-  locate(yr, g.dy, g.ny, cy, fy);
-  const long long one = 1ll << Fix<R>::QB;
-  long long ix1 = qweight(fx), iy1 = qweight(fy);
-  long long ix0 = one - ix1, iy0 = one - iy1;
-  w[0] = ix0 * iy0;
-  w[1] = ix1 * iy0;
-  w[2] = ix0 * iy1;
-  w[3] = ix1 * iy1;
-}
-
-template <typename R>
-__device__ __forceinline__ void deposit_global(unsigned long long* rho, unsigned long long* rlo,
-                                               int cx, int cy, const long long w[4],
-                                               const Geo<R>& g) {
-  const int cxp = (cx + 1 == g.nx) ? 0 : cx + 1, cyp = (cy + 1 == g.ny) ? 0 : cy + 1;
-  const long long nd[4] = {(long long)cx * g.ny + cy, (long long)cxp * g.ny + cy,
-                           (long long)cx * g.ny + cyp, (long long)cxp * g.ny + cyp};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (sizeof(R) == 8) {
-      atomicAdd(&rho[nd[k]], (unsigned long long)(w[k] >> 30));
-      atomicAdd(&rlo[nd[k]], (unsigned long long)(w[k] & ((1ll << 30) - 1)));
-    } else {
-      atomicAdd(&rho[nd[k]], (unsigned long long)w[k]);
-    }
-  }
-}
-
 // gather + kick + drift of one particle; returns synchronous-velocity sums
 // E window in LDS: ew[r*W + c] = E at node (x0 + r, y0 + c) (periodic), or NULL
 template <typename R>
@@ -588,8 +550,8 @@ __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* 
                                          double& px, double& py, const EWin<R>& w) {
   int cx, cy;
   R fx, fy;
-  locate(xr, g.dx, g.nx, cx, fx);
-  locate(yr, g.dy, g.ny, cy, fy);
+  locate_w(xr, g.idx, g.nx, cx, fx);
+  locate_w(yr, g.idy, g.ny, cy, fy);
   const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
   const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
   const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
@@ -640,9 +602,7 @@ __global__ __launch_bounds__(PIC_WG) void k_half_kick_out(const RunDesc* __restr
   if (it.x != run) return;
   const RunDesc rd = runs[run];
   const RunState rs = st[run];
-  Geo<R> g;
-  g.nx = rd.nx; g.ny = rd.ny;
-  g.dx = (R)rd.dx; g.dy = (R)rd.dy; g.Lx = (R)rd.lx; g.Ly = (R)rd.ly;
+  const Geo<R> g = make_geo<R>(rd.nx, rd.ny, rd.lx, rd.ly);
   const R* E = Eall + 2 * rd.ebase;
   const R hk = rs.vsync ? (R)0 : (R)(0.5 * rd.dt * rd.qm);
   const int p1 = min(it.y + PIC_CHUNK, rs.np);
@@ -726,9 +686,7 @@ __global__ __launch_bounds__(PIC_WG) void k_op_gather_push(R* x, R* y, R* vx, R*
                                                            double dt, double qm, double c,
                                                            double* partials) {
   __shared__ double red[4];
-  Geo<R> g;
-  g.nx = nx; g.ny = ny;
-  g.dx = (R)(lx / nx); g.dy = (R)(ly / ny); g.Lx = (R)lx; g.Ly = (R)ly;
+  const Geo<R> g = make_geo<R>(nx, ny, lx, ly);
   const R hk = (R)(c * dt * qm), dtr = (R)dt;
   const double skc = (c - 0.5) * dt * qm;
   double ke = 0, px = 0, py = 0;
@@ -737,8 +695,8 @@ __global__ __launch_bounds__(PIC_WG) void k_op_gather_push(R* x, R* y, R* vx, R*
     R xr = x[p], yr = y[p], vxr = vx[p], vyr = vy[p];
     int cx, cy;
     R fx, fy;
-    locate(xr, g.dx, nx, cx, fx);
-    locate(yr, g.dy, ny, cy, fy);
+    locate_w(xr, g.idx, nx, cx, fx);
+    locate_w(yr, g.idy, ny, cy, fy);
     const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
     const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
     const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
@@ -773,9 +731,7 @@ __global__ __launch_bounds__(PIC_WG) void k_op_deposit(const R* __restrict__ x,
                                                        int nx, int ny, double lx, double ly,
                                                        unsigned long long* hi,
                                                        unsigned long long* lo) {
-  Geo<R> g;
-  g.nx = nx; g.ny = ny;
-  g.dx = (R)(lx / nx); g.dy = (R)(ly / ny); g.Lx = (R)lx; g.Ly = (R)ly;
+  const Geo<R> g = make_geo<R>(nx, ny, lx, ly);
   for (int64_t p = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * PIC_WG) {
     int cx, cy;

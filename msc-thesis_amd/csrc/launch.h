@@ -67,7 +67,7 @@ struct PushArgs {
   int nitems;
   int nruns;
   double* partials;             // [nitems*3]
-  unsigned int* ticket;
+  unsigned int* ticket;         // per run: arrival count of its workgroups (reset by the last)
   double* diag;
   int mode;                     // 0: gather+push+deposit, 1: deposit only
   int lds_bytes;
@@ -125,7 +125,7 @@ struct FieldArgs {
   const int2* col_items; int n_col_items;
   const int2* row_items; int n_row_items;
   double* partials;                  // [n_row_items]
-  unsigned int* ticket;
+  unsigned int* ticket;         // per run: arrival count of its workgroups (reset by the last)
   double* diag;
   int nruns;
   int fp64;                          // particle precision (E type, rho words)

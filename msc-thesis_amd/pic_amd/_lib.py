@@ -133,7 +133,23 @@ SIGNATURES = {
 }
 
 
+def _share_torch_hip_runtime():
+    """Load torch's HIP runtime first when torch is installed.
+
+    torch ships its own libamdhip64 / libhsa-runtime64 with the same sonames as the system
+    ROCm ones.  Loaded first, they satisfy libpicamd.so's dependencies too, so the process
+    has one HIP runtime and torch tensors and engine buffers live in one device context.
+    Loaded second, torch would bring a second runtime into the process beside the one
+    libpicamd.so pulled in, and the second runtime can fail to open the device.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load():
+    _share_torch_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libpicamd.so not found at {LIB_PATH}; build it with __graft_entry__.build() "

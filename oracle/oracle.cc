@@ -136,6 +136,22 @@ double log_own(double x) {
 // ---------------------------------------------------------------- helpers
 inline int wrapi(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
+// Weight evaluation of the GPU mirror: t = x * inv with inv = 1/dx rounded once from the
+// FP64 cell size (one multiply instead of a division); cell (int)t and fraction t - cell,
+// both clamped.  The cell key used for sorting stays (int)(x/dx) (locate below).
+template <typename R>
+inline void locate_w(R x, R inv, int n, int* c, R* f) {
+  R t = x * inv;
+  int ci = (int)t;
+  if (ci > n - 1) ci = n - 1;
+  if (ci < 0) ci = 0;
+  R fr = t - (R)ci;
+  if (fr > (R)1) fr = (R)1;
+  if (fr < (R)0) fr = (R)0;
+  *c = ci;
+  *f = fr;
+}
+
 template <typename R>
 inline R wrap_pos(R x, R L) {
   if (x >= L) x -= L;
@@ -276,7 +292,7 @@ template <typename R, int QB>
 void push_mirror(int64_t n, R* x, R* y, R* vx, R* vy, const R* ex, const R* ey, int nx,
                  int ny, double lx, double ly, double dt, double qm, double c, double* ke,
                  double* px, double* py) {
-  const R dx = (R)(lx / nx), dy = (R)(ly / ny), Lx = (R)lx, Ly = (R)ly;
+  const R idx = (R)(1.0 / (lx / nx)), idy = (R)(1.0 / (ly / ny)), Lx = (R)lx, Ly = (R)ly;
   const R scale = (R)(1.0 / (double)(1 << QB));
   const R fq = (R)(double)(1 << QB);
   const R hk = (R)(c * dt * qm), dtr = (R)dt;
@@ -285,8 +301,8 @@ void push_mirror(int64_t n, R* x, R* y, R* vx, R* vy, const R* ex, const R* ey, 
   for (int64_t p = 0; p < n; ++p) {
     int cx, cy;
     R fx, fy;
-    locate(x[p], dx, nx, &cx, &fx);
-    locate(y[p], dy, ny, &cy, &fy);
+    locate_w(x[p], idx, nx, &cx, &fx);
+    locate_w(y[p], idy, ny, &cy, &fy);
     int ix1 = (int)std::rint(fx * fq), iy1 = (int)std::rint(fy * fq);
     R gx1 = (R)ix1 * scale, gy1 = (R)iy1 * scale;
     R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
@@ -318,10 +334,10 @@ void push_mirror(int64_t n, R* x, R* y, R* vx, R* vy, const R* ex, const R* ey, 
 }
 
 template <typename R, int QB>
-void fixed_weights(R x, R y, R dx, R dy, int nx, int ny, int* cx, int* cy, int64_t w[4]) {
+void fixed_weights(R x, R y, R idx, R idy, int nx, int ny, int* cx, int* cy, int64_t w[4]) {
   R fx, fy;
-  locate(x, dx, nx, cx, &fx);
-  locate(y, dy, ny, cy, &fy);
+  locate_w(x, idx, nx, cx, &fx);
+  locate_w(y, idy, ny, cy, &fy);
   const R fq = (R)(double)(1 << QB);
   int64_t ix1 = (int64_t)std::rint(fx * fq), iy1 = (int64_t)std::rint(fy * fq);
   int64_t ix0 = ((int64_t)1 << QB) - ix1, iy0 = ((int64_t)1 << QB) - iy1;
@@ -502,12 +518,12 @@ void or_deposit_f64(int64_t n, const double* x, const double* y, int nx, int ny,
 
 void or_deposit_fixed_f32(int64_t n, const float* x, const float* y, int nx, int ny, double lx,
                           double ly, int64_t* acc) {
-  float dx = (float)(lx / nx), dy = (float)(ly / ny);
+  const float idx = (float)(1.0 / (lx / nx)), idy = (float)(1.0 / (ly / ny));
   std::fill(acc, acc + (size_t)nx * ny, (int64_t)0);
   for (int64_t p = 0; p < n; ++p) {
     int cx, cy;
     int64_t w[4];
-    fixed_weights<float, 18>(x[p], y[p], dx, dy, nx, ny, &cx, &cy, w);
+    fixed_weights<float, 18>(x[p], y[p], idx, idy, nx, ny, &cx, &cy, w);
     int cxp = wrapi(cx + 1, nx), cyp = wrapi(cy + 1, ny);
     acc[(size_t)cx * ny + cy] += w[0];
     acc[(size_t)cxp * ny + cy] += w[1];
@@ -518,14 +534,14 @@ void or_deposit_fixed_f32(int64_t n, const float* x, const float* y, int nx, int
 
 void or_deposit_fixed_f64(int64_t n, const double* x, const double* y, int nx, int ny,
                           double lx, double ly, int64_t* hi, int64_t* lo) {
-  double dx = lx / nx, dy = ly / ny;
+  const double idx = 1.0 / (lx / nx), idy = 1.0 / (ly / ny);
   std::fill(hi, hi + (size_t)nx * ny, (int64_t)0);
   std::fill(lo, lo + (size_t)nx * ny, (int64_t)0);
   const int64_t mask = ((int64_t)1 << 30) - 1;
   for (int64_t p = 0; p < n; ++p) {
     int cx, cy;
     int64_t w[4];
-    fixed_weights<double, 30>(x[p], y[p], dx, dy, nx, ny, &cx, &cy, w);
+    fixed_weights<double, 30>(x[p], y[p], idx, idy, nx, ny, &cx, &cy, w);
     int cxp = wrapi(cx + 1, nx), cyp = wrapi(cy + 1, ny);
     size_t nd[4] = {(size_t)cx * ny + cy, (size_t)cxp * ny + cy, (size_t)cx * ny + cyp,
                     (size_t)cxp * ny + cyp};
@@ -662,13 +678,13 @@ void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy, int nx, int
     for (size_t i = 0; i < G; ++i) { exf[i] = (float)ex[i]; eyf[i] = (float)ey[i]; }
     if (s == nsteps) {
       // final half kick (mirror arithmetic, no drift)
-      const float dxf = (float)(lx / nx), dyf = (float)(ly / ny);
+      const float idx = (float)(1.0 / (lx / nx)), idy = (float)(1.0 / (ly / ny));
       const float hk = (float)(0.5 * dt * qm);
       for (int64_t p = 0; p < n; ++p) {
         int cx, cy;
         float fx, fy;
-        locate(x[p], dxf, nx, &cx, &fx);
-        locate(y[p], dyf, ny, &cy, &fy);
+        locate_w(x[p], idx, nx, &cx, &fx);
+        locate_w(y[p], idy, ny, &cy, &fy);
         int ix1 = (int)std::rint(fx * 262144.0f), iy1 = (int)std::rint(fy * 262144.0f);
         float gx1 = (float)ix1 * (1.0f / 262144.0f), gy1 = (float)iy1 * (1.0f / 262144.0f);
         float gx0 = 1.0f - gx1, gy0 = 1.0f - gy1;

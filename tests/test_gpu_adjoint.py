@@ -115,6 +115,21 @@ def test_autograd_function_matches_direct_calls():
     adj.close()
 
 
+def test_repeated_calls_replay_the_same_graphs():
+    """forward/backward run as captured graphs: replays with other inputs in between give
+    the same loss and gradients as the first call."""
+    xl, yl, vx, vy, amps = small_case(seed=17)
+    adj = Adjoint(make_cfg(32, _lib.PIC_FP64), xl, yl, default_modes(16), 12, 5)
+    first = run(adj, amps, vx, vy)
+    other = run(adj, amps * 0.5, vx[::-1].copy(), vy)
+    assert other[0] != first[0]
+    again = run(adj, amps, vx, vy)
+    assert again[0] == first[0]
+    for a, b in zip(again[1:], first[1:]):
+        assert rel_to_max(a, b) < 1e-12
+    adj.close()
+
+
 def test_adjoint_errors():
     xl, yl, vx, vy, amps = small_case()
     cfg = make_cfg(32, _lib.PIC_FP64)
