@@ -246,6 +246,20 @@ int pic_op_dsmc(float* vx, float* vy, const int32_t* id, const int32_t* offsets,
                 int ncells, float* gmax, double coeff, uint64_t seed, int64_t step,
                 int64_t* ncoll, void* stream);
 
+/* ----------------------------------------------------------------- checkpoint / restart
+ * No reference counterpart (its -f flag only truncates a file, CLd/src/main.cc:101-108).
+ * A versioned binary snapshot of one run: its config, step, velocity phase, id counter,
+ * macro-particle charge, collision counters, DSMC per-cell maxima and the particles in id
+ * order (run precision), followed by an FNV-1a checksum.  Restarting continues the run bit
+ * for bit (particle state; the diagnostics history before the restart is not stored). */
+int pic_save_checkpoint(pic_ctx* ctx, int run, const char* path);
+/* Config, particle count and completed steps stored in a checkpoint (to create a matching
+ * context: set capacity >= n). */
+int pic_checkpoint_info(const char* path, pic_config* cfg, int64_t* n, int64_t* step);
+/* Load a checkpoint into a run whose grid, domain, dt, qm, precision, collisions and seed
+ * match (PIC_ERR_INVALID_CONFIG otherwise); replaces the run's particles and state. */
+int pic_load_checkpoint(pic_ctx* ctx, int run, const char* path);
+
 /* ----------------------------------------------------------------- adjoint
  * Differentiable collisionless run (no reference counterpart).  The initial state is
  * x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl), y0 = yl, v0 given; the loss is

@@ -595,10 +595,10 @@ static int upload_particles(Ctx& x, int run, const void* px_, const void* py_, c
   return PIC_OK;
 }
 
-extern "C" {
-
-static int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void* py_,
-                              const void* pvx_, const void* pvy_, int64_t n, bool device_src) {
+namespace pic {
+int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void* py_,
+                       const void* pvx_, const void* pvy_, int64_t n, bool device_src,
+                       const Restore* restore) {
   CTX_CHECK(ctx);
   Ctx& x = ctx->impl;
   if (run < 0 || run >= x.nruns) return fail(PIC_ERR_ARG, "run index out of range");
@@ -612,7 +612,7 @@ static int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void
   RunDesc& d = x.rd[run];
   if (n > d.pcap) return fail(PIC_ERR_OVERFLOW, "more particles than the run capacity");
   const pic_config& c = x.cfg[run];
-  d.qp = (c.qp != 0.0) ? c.qp : -(c.lx * c.ly) / (double)n;
+  d.qp = restore ? restore->qp : ((c.qp != 0.0) ? c.qp : -(c.lx * c.ly) / (double)n);
   d.rho_scale = d.qp / (d.dx * d.dy) * (x.fp64 ? 1.0 : std::ldexp(1.0, -36));
   int rc = x.upload_runs();
   if (rc) return rc;
@@ -622,6 +622,11 @@ static int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void
   RunState s;
   std::memset(&s, 0, sizeof(s));
   s.np = (int)n; s.nsorted = 0; s.vsync = 1; s.step = 0; s.next_id = n;
+  if (restore) {
+    s.vsync = restore->vsync;
+    s.step = restore->step;
+    s.next_id = restore->next_id;
+  }
   PIC_HIP(hipMemcpy(x.d_st + run, &s, sizeof(s), hipMemcpyHostToDevice));
   const long long G = (long long)d.nx * d.ny;
   for (int p = 0; p < 2; ++p) {
@@ -652,14 +657,17 @@ static int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void
   x.has_particles[run] = 1;
   return PIC_OK;
 }
+}  // namespace pic
+
+extern "C" {
 
 int pic_set_particles(pic_ctx* ctx, int run, const double* x, const double* y, const double* vx,
                       const double* vy, int64_t n) {
-  return set_particles_impl(ctx, run, x, y, vx, vy, n, false);
+  return set_particles_impl(ctx, run, x, y, vx, vy, n, false, nullptr);
 }
 int pic_set_particles_device(pic_ctx* ctx, int run, const void* x, const void* y, const void* vx,
                              const void* vy, int64_t n) {
-  return set_particles_impl(ctx, run, x, y, vx, vy, n, true);
+  return set_particles_impl(ctx, run, x, y, vx, vy, n, true, nullptr);
 }
 
 }  // extern "C"

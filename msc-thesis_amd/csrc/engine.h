@@ -119,3 +119,18 @@ struct Ctx {
 struct pic_ctx {
   pic::Ctx impl;
 };
+
+namespace pic {
+// State restored with the particles (checkpoint restart): staggered velocities, step and id
+// counters, and the macro-particle charge fixed when the run started.
+struct Restore {
+  long long step;
+  int vsync;
+  long long next_id;
+  double qp;
+};
+// Uploads n particles into a run, sorts them and deposits their charge; restore == NULL
+// starts a fresh run (synchronous velocities, step 0).
+int set_particles_impl(pic_ctx* ctx, int run, const void* px, const void* py, const void* pvx,
+                       const void* pvy, int64_t n, bool device_src, const Restore* restore);
+}  // namespace pic

@@ -124,6 +124,10 @@ SIGNATURES = {
     "pic_op_gather_push": (_I, [_P, _P, _P, _P, _I64, _I, _P, _P, _GRID, _D, _D, _D, _P, _P]),
     "pic_op_mcc": (_I, [_P, _P, _P, _P, _P, ctypes.POINTER(_I64), _I64, _CFG, _I64, _P, _P]),
     "pic_op_dsmc": (_I, [_P, _P, _P, _P, _I, _P, _D, ctypes.c_uint64, _I64, _P, _P]),
+    "pic_save_checkpoint": (_I, [_P, _I, ctypes.c_char_p]),
+    "pic_checkpoint_info": (_I, [ctypes.c_char_p, _CFG, ctypes.POINTER(_I64),
+                                 ctypes.POINTER(_I64)]),
+    "pic_load_checkpoint": (_I, [_P, _I, ctypes.c_char_p]),
     "pic_adjoint_create": (_I, [_CFG, _DP, _DP, _I64, _I, ctypes.POINTER(ctypes.c_int32),
                                 ctypes.POINTER(ctypes.c_int32), _I, _I, ctypes.POINTER(_P)]),
     "pic_adjoint_destroy": (_I, [_P]),

@@ -7,6 +7,7 @@ thesisCode/collisionless/double/src/main.cc:31-194 (driver) and
 thesisCode/collisionless/double/src/pic.cu:429-611 (picFloatGpu).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -123,6 +124,40 @@ class Simulation:
                                           out.ctypes.data_as(_dp)))
         return out
 
+    CSV_HEADER = ("step,time,field_energy,kinetic_energy,total_energy,px,py,mode1,particles,"
+                  "collisions")
+
+    def write_diagnostics_csv(self, path, first=0, count=None, run=0):
+        """Write diagnostics rows as CSV (the format of bin/pic_main -f): step, time, field
+        energy, kinetic energy, their sum, momentum, |rho_hat(1,0)|/(nx ny), particle count,
+        collisions of the step.  Values are printed with 17 significant digits."""
+        d = self.diagnostics(first, count, run)
+        dt = self._run_cfg(run).dt
+        with open(path, "w") as f:
+            f.write(self.CSV_HEADER + "\n")
+            for i, r in enumerate(d):
+                s = first + i
+                f.write(f"{s},{s * dt:.9g},{r[0]:.17g},{r[1]:.17g},{r[0] + r[1]:.17g},"
+                        f"{r[2]:.17g},{r[3]:.17g},{r[4]:.17g},{r[5]:.0f},{r[6]:.0f}\n")
+
+    # ------------------------------------------------------------------ checkpoint / restart
+    def save_checkpoint(self, path, run=0):
+        """Write a restartable snapshot of one run (see pic_save_checkpoint)."""
+        check(lib.pic_save_checkpoint(self._ctx, run, os.fsencode(path)))
+
+    def load_checkpoint(self, path, run=0):
+        """Replace a run's particles and state by a snapshot of a matching run."""
+        check(lib.pic_load_checkpoint(self._ctx, run, os.fsencode(path)))
+
+    @classmethod
+    def from_checkpoint(cls, path):
+        """A new single-run context restarted from a snapshot."""
+        cfg, n, _ = checkpoint_info(path)
+        cfg.capacity = max(cfg.capacity, n)
+        sim = cls(cfg)
+        sim.load_checkpoint(path)
+        return sim
+
     def collision_counts(self, run=0):
         out = (ctypes.c_int64 * 4)()
         check(lib.pic_collision_counts(self._ctx, run, out))
@@ -176,6 +211,15 @@ def run_compat(x, y, vx, vy, nsteps, config):
     check(lib.pic_run_compat(_ptr(x), _ptr(y), _ptr(vx), _ptr(vy), _ptr(xo), _ptr(yo), n,
                              int(nsteps), ctypes.byref(config), ctypes.byref(el)))
     return xo, yo, el.value
+
+
+def checkpoint_info(path):
+    """(pic_config, particle count, completed steps) stored in a checkpoint file."""
+    cfg = _lib.pic_config()
+    n, step = ctypes.c_int64(), ctypes.c_int64()
+    check(lib.pic_checkpoint_info(os.fsencode(path), ctypes.byref(cfg), ctypes.byref(n),
+                                  ctypes.byref(step)))
+    return cfg, n.value, step.value
 
 
 # --------------------------------------------------------------------------- loaders

@@ -163,3 +163,15 @@ def test_mcc_preset_and_nu_max():
     assert nu_dt == pytest.approx(1.0001 * 1e21 * sv.max() * cfg.mcc_dt_phys * cfg.dt,
                                   rel=1e-6)
     assert nu_dt == pytest.approx(0.01, rel=1e-9)
+
+
+def test_checkpoint_info_rejects_missing_and_foreign_files(tmp_path):
+    from pic_amd import checkpoint_info
+    with pytest.raises(PicError) as e:
+        checkpoint_info(tmp_path / "missing.ckpt")
+    assert e.value.code == _lib.PIC_ERR_ARG
+    junk = tmp_path / "junk.ckpt"
+    junk.write_bytes(b"not a checkpoint" * 40)
+    with pytest.raises(PicError) as e:
+        checkpoint_info(junk)
+    assert e.value.code == _lib.PIC_ERR_ARG
