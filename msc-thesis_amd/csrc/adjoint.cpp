@@ -37,6 +37,7 @@ struct pic_adjoint {
   RunState* st = nullptr;
   int2 *rp = nullptr, *cl = nullptr, *rw = nullptr;
   int nrp = 0, ncl = 0, nrw = 0;
+  int row_lds = 0;  // complex values per row-pass workgroup
   double* part = nullptr;
   bool forward_done = false;
   hipGraphExec_t gfwd = nullptr, gbwd = nullptr;  // captured step loops (made on first use)
@@ -82,6 +83,7 @@ void field(pic_adjoint& a, void* E) {
   f.col_items = a.cl; f.n_col_items = a.ncl;
   f.row_items = a.rw; f.n_row_items = a.nrw;
   f.nruns = 1; f.fp64 = a.fp64; f.want_phi = 0; f.max_nx = a.g.nx; f.max_ny = a.g.ny;
+  f.row_lds = a.row_lds;
   f.rho_in = a.rho;
   launch_field(a.s, f, nullptr);
 }
@@ -285,15 +287,17 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   std::memset(&d, 0, sizeof(d));
   d.nx = g.nx; d.ny = g.ny; d.lognx = g.lognx; d.logny = g.logny;
   d.lx = g.lx; d.ly = g.ly; d.dx = g.dx; d.dy = g.dy;
-  d.row_item_begin = 0; d.row_item_end = g.nx;
+  fft_groups(g.nx, g.nx, g.ny, &d.fft_pp, &d.fft_rf);
+  A.row_lds = std::max(d.fft_pp, d.fft_rf) * g.ny;
+  d.row_item_begin = 0; d.row_item_end = g.nx / d.fft_rf;
   PIC_HIP(hipMemcpy(A.dd, &d, sizeof(d), hipMemcpyHostToDevice));
   PIC_HIP(hipMemset(A.st, 0, sizeof(RunState)));
   PIC_HIP(hipMemset(A.hi, 0, sizeof(unsigned long long) * G));
   PIC_HIP(hipMemset(A.lo, 0, sizeof(unsigned long long) * G));
   std::vector<int2> rp, cl, rw;
-  for (int i = 0; i < g.nx; i += 2) rp.push_back(make_int2(0, i));
+  for (int i = 0; i < g.nx; i += 2 * d.fft_pp) rp.push_back(make_int2(0, i));
   for (int k = 0; k <= g.ny / 2; ++k) cl.push_back(make_int2(0, k));
-  for (int i = 0; i < g.nx; ++i) rw.push_back(make_int2(0, i));
+  for (int i = 0; i < g.nx; i += d.fft_rf) rw.push_back(make_int2(0, i));
   PIC_HIP(A.get(&A.rp, rp.size()));
   PIC_HIP(A.get(&A.cl, cl.size()));
   PIC_HIP(A.get(&A.rw, rw.size()));

@@ -128,6 +128,8 @@ int Ctx::build_items() {
   lds_push = 0;
   long long tsum = 0, toffsum = 0;
   int maxb = 1;
+  long long rows_total = 0;  // rows of all runs: sets the FFT row grouping
+  for (int r = 0; r < nruns; ++r) rows_total += cfg[r].nx;
   for (int r = 0; r < nruns; ++r) {
     RunDesc& d = rd[r];
     const Tuning& t = tune[r];
@@ -163,10 +165,11 @@ int Ctx::build_items() {
     if (cfg[r].collide == PIC_COLLIDE_MCC)
       for (long long s = 0; s < preg; s += PIC_CHUNK) push_items.push_back(make_int4(r, -1, (int)s, 0));
     d.push_item_end = (int)push_items.size();
-    for (int i = 0; i < d.nx; i += 2) rowpair.push_back(make_int2(r, i));
+    fft_groups(rows_total, d.nx, d.ny, &d.fft_pp, &d.fft_rf);
+    for (int i = 0; i < d.nx; i += 2 * d.fft_pp) rowpair.push_back(make_int2(r, i));
     for (int k = 0; k <= d.ny / 2; ++k) cols.push_back(make_int2(r, k));
     d.row_item_begin = (int)rows.size();
-    for (int i = 0; i < d.nx; ++i) rows.push_back(make_int2(r, i));
+    for (int i = 0; i < d.nx; i += d.fft_rf) rows.push_back(make_int2(r, i));
     d.row_item_end = (int)rows.size();
   }
   if (lds_push > 160 * 1024) return fail(PIC_ERR_INVALID_CONFIG, "deposit window exceeds LDS");
@@ -309,6 +312,8 @@ FieldArgs Ctx::field_args(int want_phi) const {
   a.partials = partials_rows; a.ticket = tickets + nruns; a.diag = diag;
   a.nruns = nruns; a.fp64 = fp64; a.want_phi = want_phi;
   a.max_nx = max_nx; a.max_ny = max_ny;
+  a.row_lds = 0;
+  for (const RunDesc& d : rd) a.row_lds = std::max(a.row_lds, std::max(d.fft_pp, d.fft_rf) * d.ny);
   a.rho_in = nullptr;
   return a;
 }

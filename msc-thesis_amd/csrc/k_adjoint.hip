@@ -314,8 +314,7 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_cols(AdjGrid g, double2* __restr
     b[bitrev(i, g.lognx)] = A[(long long)k2 * nx + i];
   }
   __syncthreads();
-  fft_lds(a, nx, g.lognx, -1, tw);
-  fft_lds(b, nx, g.lognx, -1, tw);
+  fft_lds_rows(a, 2, nx, g.lognx, -1, tw);  // a, b consecutive
   const double twopi = 6.283185307179586476925;
   for (int m = threadIdx.x; m < nx; m += PIC_WG) {
     const int m2 = (nx - m) & (nx - 1);
@@ -340,8 +339,7 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_cols(AdjGrid g, double2* __restr
     }
   }
   __syncthreads();
-  fft_lds(ra, nx, g.lognx, +1, tw);
-  fft_lds(rb, nx, g.lognx, +1, tw);
+  fft_lds_rows(ra, 2, nx, g.lognx, +1, tw);  // ra, rb consecutive
   for (int i = threadIdx.x; i < nx; i += PIC_WG) {
     A[(long long)k * nx + i] = ra[i];
     if (k2 != k) A[(long long)k2 * nx + i] = rb[i];

@@ -4,13 +4,15 @@
 //   cufftExecD2Z :551, fftPoissonSolver :147-194, cufftExecZ2D :557,
 //   normaliseTransform :218-232, copyD2D :203-210, computeElectricField :242-283.
 // Three kernels per step:
-//   rows    : fixed-point rho (two rows packed as one complex row) -> row spectra,
-//             written transposed as S[ky][x]; clears the other rho parity buffer
+//   rows    : fixed-point rho (two rows packed as one complex row, groups of row pairs
+//             per workgroup) -> row spectra, written transposed as S[ky][x]; clears the
+//             other rho parity buffer
 //   columns : column FFT, phi_hat = rho_hat / |k|^2 (k = 2 pi m / L, mean removed),
 //             Ex_hat = -i sin(kx dx)/dx phi_hat, Ey_hat likewise (the Fourier symbol of the
 //             periodic central difference), inverse column FFTs
-//   field   : Ex + i Ey from one inverse complex row FFT per row; writes the padded
-//             (Ex, Ey) node array and the field energy (fixed-order reduction)
+//   field   : Ex + i Ey from one inverse complex row FFT per row (groups of rows per
+//             workgroup); writes the padded (Ex, Ey) node array and the field energy
+//             (fixed-order reduction)
 #include <type_traits>
 #include "fft_lds.h"
 #include "launch.h"
@@ -25,41 +27,53 @@ __device__ __forceinline__ double fixed_to_density(unsigned long long hi, unsign
   return (double)(long long)hi * scale;
 }
 
+__device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
+
+// Group of PP row pairs (rows i0 .. i0 + 2PP - 1): pair q is one complex FFT of
+// row i0+2q + i row i0+2q+1; the half spectra are written to S[k][i0 .. i0+2PP) with
+// consecutive threads on consecutive rows.
 __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   extern __shared__ double2 lds[];
   const int2 it = a.row_pair_items[blockIdx.x];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
+  const int PP = rd.fft_pp;
   const int i0 = it.y;
-  const long long g0 = rd.gbase + (long long)i0 * ny, g1 = g0 + ny;
+  const long long g0 = rd.gbase + (long long)i0 * ny;
   if (a.rho_in) {
     const double* r = a.rho_in + (g0 - rd.gbase);
-    for (int j = threadIdx.x; j < ny; j += PIC_WG)
-      lds[bitrev(j, logny)] = make_double2(r[j], r[j + ny]);
+    for (int t = threadIdx.x; t < PP * ny; t += PIC_WG) {
+      const int q = t >> logny, j = t & (ny - 1);
+      lds[q * ny + bitrev(j, logny)] =
+          make_double2(r[(long long)(2 * q) * ny + j], r[(long long)(2 * q + 1) * ny + j]);
+    }
   } else {
     const int par = (int)(a.st[it.x].step & 1);
     const unsigned long long* rh = a.rho + par * a.gsum;
     unsigned long long* zw = a.rho_w + (1 - par) * a.gsum;
     const unsigned long long* rl = a.fp64 ? a.rho_lo + par * a.gsum : nullptr;
     unsigned long long* zl = a.fp64 ? a.rho_lo_w + (1 - par) * a.gsum : nullptr;
-    for (int j = threadIdx.x; j < ny; j += PIC_WG) {
-      const double d0 = fixed_to_density(rh[g0 + j], a.fp64 ? rl[g0 + j] : 0ull, a.fp64, rd.rho_scale);
-      const double d1 = fixed_to_density(rh[g1 + j], a.fp64 ? rl[g1 + j] : 0ull, a.fp64, rd.rho_scale);
-      lds[bitrev(j, logny)] = make_double2(d0, d1);
-      zw[g0 + j] = 0ull;
-      zw[g1 + j] = 0ull;
-      if (a.fp64) { zl[g0 + j] = 0ull; zl[g1 + j] = 0ull; }
+    for (int t = threadIdx.x; t < PP * ny; t += PIC_WG) {
+      const int q = t >> logny, j = t & (ny - 1);
+      const long long e0 = g0 + (long long)(2 * q) * ny + j, e1 = e0 + ny;
+      const double d0 = fixed_to_density(rh[e0], a.fp64 ? rl[e0] : 0ull, a.fp64, rd.rho_scale);
+      const double d1 = fixed_to_density(rh[e1], a.fp64 ? rl[e1] : 0ull, a.fp64, rd.rho_scale);
+      lds[q * ny + bitrev(j, logny)] = make_double2(d0, d1);
+      zw[e0] = 0ull;
+      zw[e1] = 0ull;
+      if (a.fp64) { zl[e0] = 0ull; zl[e1] = 0ull; }
     }
   }
   __syncthreads();
-  fft_lds(lds, ny, logny, -1, a.tw);
+  fft_lds_rows(lds, PP, ny, logny, -1, a.tw);
   double2* S = a.S + rd.sbase;
-  for (int k = threadIdx.x; k <= ny / 2; k += PIC_WG) {
-    const double2 zk = lds[k], zm = lds[(ny - k) & (ny - 1)];
-    const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-    const double2 B = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-    S[(long long)k * nx + i0] = A;
-    S[(long long)k * nx + i0 + 1] = B;
+  const int lg = ilog2(2 * PP);
+  for (int t = threadIdx.x; t < (ny / 2 + 1) * 2 * PP; t += PIC_WG) {
+    const int rr = t & (2 * PP - 1), k = t >> lg, q = rr >> 1;
+    const double2 zk = lds[q * ny + k], zm = lds[q * ny + ((ny - k) & (ny - 1))];
+    const double2 v = (rr & 1) ? make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x))
+                               : make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+    S[(long long)k * nx + i0 + rr] = v;
   }
 }
 
@@ -101,9 +115,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
     if (a.want_phi) C[br] = ph;
   }
   __syncthreads();
-  fft_lds(A, nx, lognx, +1, a.tw);
-  fft_lds(B, nx, lognx, +1, a.tw);
-  if (a.want_phi) fft_lds(C, nx, lognx, +1, a.tw);
+  fft_lds_rows(A, a.want_phi ? 3 : 2, nx, lognx, +1, a.tw);  // A, B, C are consecutive
   double2* Sx = a.Sx + rd.sbase + (long long)k * nx;
   double2* Sy = a.Sy + rd.sbase + (long long)k * nx;
   for (int i = threadIdx.x; i < nx; i += PIC_WG) {
@@ -114,6 +126,9 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
 }
 
 // MODE 0: row i of (Ex + i Ey) -> padded E + field energy.  MODE 1: rows (i, i+1) of phi.
+// MODE 0: group of RF rows of (Ex + i Ey), one inverse complex FFT per row -> padded E and
+// the field energy.  MODE 1: group of PP row pairs of phi (two real rows per FFT).  The
+// spectra are read along x (consecutive threads on consecutive rows).
 template <typename R, int MODE>
 __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   extern __shared__ double2 lds[];
@@ -122,40 +137,47 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   const int2 it = (MODE == 0) ? a.row_items[blockIdx.x] : a.row_pair_items[blockIdx.x];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
-  const int i = it.y;
-  const double2* SA = (MODE == 0) ? a.Sx + rd.sbase + i : a.S + rd.sbase + i;
-  const double2* SB = (MODE == 0) ? a.Sy + rd.sbase + i : a.S + rd.sbase + i + 1;
-  for (int k = threadIdx.x; k < ny; k += PIC_WG) {
+  const int i0 = it.y;
+  // rows held in LDS (MODE 0: RF rows; MODE 1: PP packed pairs) and rows read per row slot
+  const int nr = (MODE == 0) ? rd.fft_rf : rd.fft_pp;
+  const int lg = ilog2(nr);
+  const double2* SA = (MODE == 0) ? a.Sx + rd.sbase + i0 : a.S + rd.sbase + i0;
+  const double2* SB = (MODE == 0) ? a.Sy + rd.sbase + i0 : a.S + rd.sbase + i0 + 1;
+  for (int t = threadIdx.x; t < nr * ny; t += PIC_WG) {
+    const int rr = t & (nr - 1), k = t >> lg;
+    const int col = (MODE == 0) ? rr : 2 * rr;  // offset of row slot rr from i0
     double2 A, B;
     if (2 * k <= ny) {
-      A = SA[(long long)k * nx];
-      B = SB[(long long)k * nx];
+      A = SA[(long long)k * nx + col];
+      B = SB[(long long)k * nx + col];
       if (k == 0 || 2 * k == ny) { A.y = 0.0; B.y = 0.0; }
     } else {
       const int kk = ny - k;
-      A = SA[(long long)kk * nx];
-      B = SB[(long long)kk * nx];
+      A = SA[(long long)kk * nx + col];
+      B = SB[(long long)kk * nx + col];
       A.y = -A.y;
       B.y = -B.y;
     }
-    lds[bitrev(k, logny)] = make_double2(A.x - B.y, A.y + B.x);
+    lds[rr * ny + bitrev(k, logny)] = make_double2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  fft_lds(lds, ny, logny, +1, a.tw);
+  fft_lds_rows(lds, nr, ny, logny, +1, a.tw);
   const double norm = 1.0 / ((double)nx * ny);
   if (MODE == 1) {
     double* ph = a.phi + rd.gbase;
-    for (int j = threadIdx.x; j < ny; j += PIC_WG) {
-      ph[(long long)i * ny + j] = lds[j].x * norm;
-      ph[(long long)(i + 1) * ny + j] = lds[j].y * norm;
+    for (int t = threadIdx.x; t < nr * ny; t += PIC_WG) {
+      const int q = t >> logny, j = t & (ny - 1);
+      ph[(long long)(i0 + 2 * q) * ny + j] = lds[q * ny + j].x * norm;
+      ph[(long long)(i0 + 2 * q + 1) * ny + j] = lds[q * ny + j].y * norm;
     }
     return;
   }
   typedef typename std::conditional<sizeof(R) == 4, float2, double2>::type R2;
   R2* E = reinterpret_cast<R2*>(a.E) + rd.ebase;
   double en = 0.0;
-  for (int j = threadIdx.x; j < ny; j += PIC_WG) {
-    const double ex = lds[j].x * norm, ey = lds[j].y * norm;
+  for (int t = threadIdx.x; t < nr * ny; t += PIC_WG) {
+    const int rr = t >> logny, j = t & (ny - 1), i = i0 + rr;
+    const double ex = lds[rr * ny + j].x * norm, ey = lds[rr * ny + j].y * norm;
     en += ex * ex + ey * ey;
     R2 v;
     v.x = (R)ex;
@@ -175,7 +197,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
     ticket_s = atomicAdd(a.ticket + it.x, 1u);
   }
   __syncthreads();
-  // the last row of this run sums the run's row partials in row order
+  // the last row group of this run sums the run's partials in row order
   if (ticket_s != (unsigned int)(rd.row_item_end - rd.row_item_begin - 1)) return;
   __threadfence();
   double v = 0.0;
@@ -243,7 +265,7 @@ inline int grid_for(long long n) {
 }  // namespace
 
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev) {
-  const size_t lrow = (size_t)a.max_ny * sizeof(double2);
+  const size_t lrow = (size_t)a.row_lds * sizeof(double2);
   const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3);
   if (ev) (void)hipEventRecord(ev[0], s);
   k_fft_rows<<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);

@@ -107,6 +107,34 @@ void launch_fill_u64(hipStream_t s, unsigned long long* p, long long n);
 void launch_reduce_sum(hipStream_t s, const double* parts, int n, double* out);
 
 // ---- field kernels (k_field.hip)
+// Row passes work on groups of consecutive rows (at most about 4096 complex values of LDS),
+// so the transposed spectra S[k][x] are read and written in contiguous runs along x.
+// Forward rows are packed in pairs (two real rows = one complex FFT).
+__host__ __device__ inline int fft_row_pairs_per_block(int nx, int ny) {
+  int p = 4096 / ny;
+  p = p < 1 ? 1 : (p > 8 ? 8 : p);
+  return p > nx / 2 ? nx / 2 : p;
+}
+__host__ __device__ inline int fft_rows_per_block(int nx, int ny) {
+  int r = 4096 / ny;
+  r = r < 1 ? 1 : (r > 16 ? 16 : r);
+  return r > nx ? nx : r;
+}
+// Group sizes used by a launch: about one workgroup per CU (256) over all runs' rows, never
+// above the per-grid limits.  A single large grid keeps one pair / row per workgroup (its
+// rows alone fill the chip); many small grids get grouped rows.
+inline int pow2_floor(long long v) {
+  int p = 1;
+  while ((long long)p * 2 <= v) p *= 2;
+  return p;
+}
+inline void fft_groups(long long rows_total, int nx, int ny, int* pp, int* rf) {
+  const long long t = rows_total / 256, tp = rows_total / 512;
+  const int gr = t >= 1 ? pow2_floor(t) : 1, gp = tp >= 1 ? pow2_floor(tp) : 1;
+  *rf = fft_rows_per_block(nx, ny) < gr ? fft_rows_per_block(nx, ny) : gr;
+  *pp = fft_row_pairs_per_block(nx, ny) < gp ? fft_row_pairs_per_block(nx, ny) : gp;
+}
+
 struct FieldArgs {
   const RunDesc* runs;
   RunState* st;
@@ -131,6 +159,8 @@ struct FieldArgs {
   int fp64;                          // particle precision (E type, rho words)
   int want_phi;
   int max_nx, max_ny;                // largest grid of the context (LDS sizing)
+  int row_lds;                       // complex values per row-pass workgroup: the largest
+                                     // max(fft_pp, fft_rf) * ny over the runs
   const double* rho_in;              // op mode: FP64 density input instead of rho (may be NULL)
 };
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev /* 4 events or NULL */);

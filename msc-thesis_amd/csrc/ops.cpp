@@ -49,6 +49,7 @@ RunDesc op_desc(int nx, int ny, double lx, double ly, long long pcap) {
   d.logny = l;
   d.lx = lx; d.ly = ly; d.dx = lx / nx; d.dy = ly / ny;
   d.pcap = (int)pcap;
+  fft_groups(nx, nx, ny, &d.fft_pp, &d.fft_rf);
   return d;
 }
 
@@ -174,7 +175,7 @@ int pic_op_poisson(const double* rho, double* phi, const pic_grid* g, void* stre
   DevMem m;
   RunDesc d = op_desc(nx, ny, g->lx, g->ly, 0);
   d.row_item_begin = 0;
-  d.row_item_end = nx;
+  d.row_item_end = nx / d.fft_rf;
   RunDesc* dd;
   RunState* st;
   PIC_HIP(m.get(&dd, 1));
@@ -196,9 +197,9 @@ int pic_op_poisson(const double* rho, double* phi, const pic_grid* g, void* stre
   }
   PIC_HIP(hipMemcpyAsync(tw, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, s));
   std::vector<int2> rp, cl, rw;
-  for (int i = 0; i < nx; i += 2) rp.push_back(make_int2(0, i));
+  for (int i = 0; i < nx; i += 2 * d.fft_pp) rp.push_back(make_int2(0, i));
   for (int k = 0; k <= ny / 2; ++k) cl.push_back(make_int2(0, k));
-  for (int i = 0; i < nx; ++i) rw.push_back(make_int2(0, i));
+  for (int i = 0; i < nx; i += d.fft_rf) rw.push_back(make_int2(0, i));
   int2 *drp, *dcl, *drw;
   PIC_HIP(m.get(&drp, rp.size()));
   PIC_HIP(m.get(&dcl, cl.size()));
@@ -214,6 +215,7 @@ int pic_op_poisson(const double* rho, double* phi, const pic_grid* g, void* stre
   a.col_items = dcl; a.n_col_items = (int)cl.size();
   a.row_items = drw; a.n_row_items = (int)rw.size();
   a.nruns = 1; a.fp64 = 0; a.want_phi = 1; a.max_nx = nx; a.max_ny = ny;
+  a.row_lds = std::max(d.fft_pp, d.fft_rf) * ny;
   a.rho_in = rho;
   launch_field(s, a, nullptr);
   PIC_HIP(hipGetLastError());

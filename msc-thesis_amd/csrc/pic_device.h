@@ -26,10 +26,11 @@ struct RunDesc {
   int push_item_begin, push_item_end;
   int row_item_begin, row_item_end;  // field-kernel row items of this run
   int cell_item_begin, cell_item_end;
-  int pcap, pad0;
+  int pcap, fft_pp;                  // fft_pp: row pairs per workgroup of the forward rows
   int nbuckets, nchunks;             // tiles (sort buckets) and PIC_CHUNK particle chunks
   int tscan_item_begin, tscan_item_end;
-  int key_tile_major, pad1;          // 1: cell keys ordered tile-major (engine sorts)
+  int key_tile_major;                // 1: cell keys ordered tile-major (engine sorts)
+  int fft_rf;                        // rows per workgroup of the inverse (field) rows
   long long tbase;                   // bucket x chunk count table base (nbuckets*nchunks)
   long long toffbase;                // tile offsets base (nbuckets + 1)
   double lx, ly, dx, dy;
