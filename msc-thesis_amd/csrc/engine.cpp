@@ -100,7 +100,8 @@ Ctx::~Ctx() {
   for (int i = 0; i < 2; ++i) { F(px[i]); F(py[i]); F(pvx[i]); F(pvy[i]); F(pid[i]); }
   F(rho); F(rho_lo); F(E); F(S); F(Sx); F(Sy); F(phi); F(tw);
   F(sb.keys); F(sb.rank); F(sb.perm); F(sb.counts); F(sb.offsets); F(sb.chunk_sums);
-  F(sb.big_list); F(sb.big_count); F(sb.scratch); F(sb.table); F(sb.tchunk_sums); F(sb.toff);
+  F(sb.big_list); F(sb.big_count); F(sb.scratch); F(sb.table); F(sb.tchunk_sums);
+  F(toffs[0]); F(toffs[1]); F(sb.ftable); F(sb.fchunk_sums); F(d_fscan_items);
   F(d_pitems); F(d_citems); F(d_rowpair); F(d_cols); F(d_rows); F(d_push);
   F(d_segitems); F(d_tscan_items);
   F(partials_push); F(partials_rows); F(tickets); F(diag);
@@ -165,6 +166,7 @@ int Ctx::build_items() {
     if (cfg[r].collide == PIC_COLLIDE_MCC)
       for (long long s = 0; s < preg; s += PIC_CHUNK) push_items.push_back(make_int4(r, -1, (int)s, 0));
     d.push_item_end = (int)push_items.size();
+    d.fchunks = 0;
     fft_groups(rows_total, d.nx, d.ny, &d.fft_pp, &d.fft_rf);
     for (int i = 0; i < d.nx; i += 2 * d.fft_pp) rowpair.push_back(make_int2(r, i));
     for (int k = 0; k <= d.ny / 2; ++k) cols.push_back(make_int2(r, k));
@@ -173,6 +175,26 @@ int Ctx::build_items() {
     d.row_item_end = (int)rows.size();
   }
   if (lds_push > 160 * 1024) return fail(PIC_ERR_INVALID_CONFIG, "deposit window exceeds LDS");
+  // Sort fused into the push: needs no births or cell lists (no collisions), a cadence of
+  // at least two steps (the count is taken one step ahead), and room in LDS for one
+  // counter per tile and wave.
+  const int fused_lds = PUSH_NW * maxb * (int)sizeof(int);
+  fused = !any_mcc && !any_dsmc && sort_every >= 2 && maxb <= 512 &&
+          lds_push + fused_lds <= 160 * 1024;
+  fscan_items.clear();
+  long long fsum = 0;
+  for (int r = 0; r < nruns && fused; ++r) {
+    RunDesc& d = rd[r];
+    d.fchunks = (d.push_item_end - d.push_item_begin) * PUSH_NW;
+    d.ftbase = fsum;
+    fsum += (long long)d.nbuckets * d.fchunks;
+    d.fscan_item_begin = (int)fscan_items.size();
+    for (long long s = 0; s < (long long)d.nbuckets * d.fchunks; s += PIC_CELL_CHUNK)
+      fscan_items.push_back(make_int2(r, (int)s));
+    d.fscan_item_end = (int)fscan_items.size();
+  }
+  if (fsum >= (1ll << 31)) fused = false;
+  if (fused) lds_push += fused_lds;
   auto up = [&](auto& vec, auto** dptr) -> int {
     using T = typename std::remove_reference<decltype(vec)>::type::value_type;
     if (*dptr) (void)hipFree(*dptr);
@@ -190,6 +212,13 @@ int Ctx::build_items() {
   if ((rc = up(push_items, &d_push))) return rc;
   if ((rc = up(segitems, &d_segitems))) return rc;
   if ((rc = up(tscan_items, &d_tscan_items))) return rc;
+  if ((rc = up(fscan_items, &d_fscan_items))) return rc;
+  sb.fscan_items = d_fscan_items;
+  sb.nfscan_items = (int)fscan_items.size();
+  if (sb.ftable) (void)hipFree(sb.ftable);
+  PIC_HIP(dalloc(&sb.ftable, fused ? fsum : 1));
+  if (sb.fchunk_sums) (void)hipFree(sb.fchunk_sums);
+  PIC_HIP(dalloc(&sb.fchunk_sums, fscan_items.size()));
   sb.segitems = d_segitems;
   sb.nsegitems = (int)segitems.size();
   sb.tscan_items = d_tscan_items;
@@ -199,9 +228,12 @@ int Ctx::build_items() {
   PIC_HIP(dalloc(&sb.table, tsum));
   if (sb.tchunk_sums) (void)hipFree(sb.tchunk_sums);
   PIC_HIP(dalloc(&sb.tchunk_sums, tscan_items.size()));
-  if (sb.toff) (void)hipFree(sb.toff);
-  PIC_HIP(dalloc(&sb.toff, toffsum));
-  PIC_HIP(hipMemset(sb.toff, 0, sizeof(int) * toffsum));
+  for (int k = 0; k < 2; ++k) {
+    if (toffs[k]) (void)hipFree(toffs[k]);
+    PIC_HIP(dalloc(&toffs[k], toffsum));
+    PIC_HIP(hipMemset(toffs[k], 0, sizeof(int) * toffsum));
+  }
+  sb.toff = toffs[1 - cur];
   if (partials_push) (void)hipFree(partials_push);
   PIC_HIP(dalloc(&partials_push, push_items.size() * 3));
   if (partials_rows) (void)hipFree(partials_rows);
@@ -319,7 +351,7 @@ FieldArgs Ctx::field_args(int want_phi) const {
 }
 
 template <typename R>
-PushArgs<R> Ctx::push_args(int mode) const {
+PushArgs<R> Ctx::push_args(int mode, int sort_mode) const {
   PushArgs<R> a;
   std::memset(&a, 0, sizeof(a));
   a.runs = d_runs; a.st = d_st;
@@ -327,11 +359,15 @@ PushArgs<R> Ctx::push_args(int mode) const {
   a.x = b.x; a.y = b.y; a.vx = b.vx; a.vy = b.vy;
   a.E = (const R*)E;
   a.rho = rho; a.rho_lo = rho_lo; a.gsum = gsum;
-  a.off = sb.toff;
+  a.off = toffs[cur];
   a.items = d_push; a.nitems = (int)push_items.size();
   a.nruns = nruns;
   a.partials = partials_push; a.ticket = tickets; a.diag = diag;
   a.mode = mode; a.lds_bytes = lds_push;
+  a.sort_mode = fused ? sort_mode : 0;
+  a.ftable = sb.ftable;
+  a.dst = bufs<R>(1 - cur);
+  a.id = pid[cur];
   return a;
 }
 
@@ -341,13 +377,14 @@ int Ctx::launch_field_stage(int want_phi, hipEvent_t* ev4) {
   return PIC_OK;
 }
 
-int Ctx::launch_push_stage(int mode) {
-  if (fp64) launch_push<double>(stream, push_args<double>(mode));
-  else launch_push<float>(stream, push_args<float>(mode));
+int Ctx::launch_push_stage(int mode, int sort_mode) {
+  if (fp64) launch_push<double>(stream, push_args<double>(mode, sort_mode));
+  else launch_push<float>(stream, push_args<float>(mode, sort_mode));
   return PIC_OK;
 }
 
 int Ctx::launch_sort_stage() {
+  sb.toff = toffs[1 - cur];  // offsets of the order being written
   if (!any_dsmc) {
     // tile sort: the step only needs tiles to be contiguous
     if (fp64)
@@ -419,15 +456,34 @@ int Ctx::launch_collide_stage() {
   return PIC_OK;
 }
 
-int Ctx::launch_one_step(bool allow_sort) {
-  launch_field_stage(0);
-  launch_push_stage(0);
+int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev) {
+  launch_field_stage(0, ev);
+  // fused sort: the push one step before the sort step counts the tiles of the positions
+  // it produces (then the counts are scanned), the push of the sort step scatters
+  int smode = 0;
+  if (fused && allow_sort) {
+    if (steps_since_sort == sort_every - 2) smode = 1;
+    else if (steps_since_sort == sort_every - 1) smode = 2;
+  }
+  launch_push_stage(0, smode);
+  if (smode == 1) {
+    sb.toff = toffs[1 - cur];
+    launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
+  }
+  if (ev) (void)hipEventRecord(ev[4], stream);
   launch_collide_stage();
+  if (ev) (void)hipEventRecord(ev[5], stream);
   if (any_dsmc) {
     steps_since_sort = 0;
-  } else if (allow_sort && ++steps_since_sort >= sort_every) {
-    launch_sort_stage();
-    steps_since_sort = 0;
+  } else if (allow_sort) {
+    ++steps_since_sort;
+    if (smode == 2) {
+      cur ^= 1;
+      steps_since_sort = 0;
+    } else if (!fused && steps_since_sort >= sort_every) {
+      launch_sort_stage();
+      steps_since_sort = 0;
+    }
   }
   return PIC_OK;
 }
@@ -775,8 +831,10 @@ int pic_sync(pic_ctx* ctx) {
   PIC_HIP(hipStreamSynchronize(x.stream));
   std::vector<RunState> s(x.nruns);
   PIC_HIP(hipMemcpy(s.data(), x.d_st, sizeof(RunState) * x.nruns, hipMemcpyDeviceToHost));
-  for (int r = 0; r < x.nruns; ++r)
+  for (int r = 0; r < x.nruns; ++r) {
     if (s[r].err & 1) return fail(PIC_ERR_OVERFLOW, "particle capacity exceeded (births dropped)");
+    if (s[r].err & 2) return fail(PIC_ERR_STATE, "sort slot outside the run (tile counts out of date)");
+  }
   return PIC_OK;
 }
 
@@ -1025,16 +1083,7 @@ int pic_stage_times(pic_ctx* ctx, int nsteps, double* out) {
     if (!x.has_particles[r]) return fail(PIC_ERR_STATE, "run has no particles");
   double acc[PIC_NSTAGES] = {0, 0, 0, 0, 0, 0};
   for (int s = 0; s < nsteps; ++s) {
-    x.launch_field_stage(0, x.ev);
-    x.launch_push_stage(0);
-    PIC_HIP(hipEventRecord(x.ev[4], x.stream));
-    x.launch_collide_stage();
-    PIC_HIP(hipEventRecord(x.ev[5], x.stream));
-    if (!x.any_dsmc && ++x.steps_since_sort >= x.sort_every) {
-      x.launch_sort_stage();
-      x.steps_since_sort = 0;
-    }
-    if (x.any_dsmc) x.steps_since_sort = 0;
+    x.launch_one_step(true, x.ev);
     PIC_HIP(hipEventRecord(x.ev[6], x.stream));
     PIC_HIP(hipEventSynchronize(x.ev[6]));
     float ms[6];

@@ -65,10 +65,13 @@ struct Ctx {
   double* phi = nullptr;
   double2* tw = nullptr;
   SortBufs sb{};
-  std::vector<int2> pitems, citems, rowpair, cols, rows, segitems, tscan_items;
+  int* toffs[2] = {nullptr, nullptr};  // tile offsets of the order in buffer cur / 1 - cur
+  bool fused = false;                  // sort fused into the push (see build_items)
+  std::vector<int2> pitems, citems, rowpair, cols, rows, segitems, tscan_items, fscan_items;
   std::vector<int4> push_items;
   int2 *d_pitems = nullptr, *d_citems = nullptr, *d_rowpair = nullptr, *d_cols = nullptr,
-       *d_rows = nullptr, *d_segitems = nullptr, *d_tscan_items = nullptr;
+       *d_rows = nullptr, *d_segitems = nullptr, *d_tscan_items = nullptr,
+       *d_fscan_items = nullptr;
   int4* d_push = nullptr;
   double* partials_push = nullptr;
   double* partials_rows = nullptr;
@@ -101,15 +104,16 @@ struct Ctx {
     return b;
   }
   FieldArgs field_args(int want_phi) const;
-  template <typename R> PushArgs<R> push_args(int mode) const;
+  template <typename R> PushArgs<R> push_args(int mode, int sort_mode = 0) const;
 
   // launches (no synchronisation)
   int launch_field_stage(int want_phi, hipEvent_t* ev4 = nullptr);
-  int launch_push_stage(int mode);
+  int launch_push_stage(int mode, int sort_mode = 0);
   int launch_sort_stage();
   int launch_collide_stage();
   double numax_dt_of(int r);
-  int launch_one_step(bool allow_sort);
+  // one step; ev (7 events or NULL) brackets the stages for pic_stage_times
+  int launch_one_step(bool allow_sort, hipEvent_t* ev = nullptr);
   int step(int nsteps);
   int capture_macro(int which);
 };

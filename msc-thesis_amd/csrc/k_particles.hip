@@ -385,14 +385,36 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
   for (int b = threadIdx.x; b < nb; b += PIC_WG) h[b] = 0;
   __syncthreads();
   const int p1 = min(it.y + PIC_CHUNK, np);
-  for (int p = it.y + threadIdx.x; p < p1; p += PIC_WG)
-    atomicAdd(&h[bucket_of<R>(x[rd.pbase + p], y[rd.pbase + p], rd)], 1);
+  // four particles per thread per pass, loads first (PIC_CHUNK = 16 * PIC_WG)
+  for (int q = it.y + threadIdx.x; q < p1; q += 4 * PIC_WG) {
+    R xs[4], ys[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int p = q + u * PIC_WG;
+      xs[u] = p < p1 ? x[rd.pbase + p] : (R)0;
+      ys[u] = p < p1 ? y[rd.pbase + p] : (R)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (q + u * PIC_WG < p1) atomicAdd(&h[bucket_of<R>(xs[u], ys[u], rd)], 1);
+  }
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += PIC_WG)
     table[rd.tbase + (long long)b * rd.nchunks + chunk] = h[b];
 }
 
+// table of a run: the tile sort's (FUSED = false) or the fused sort's
+template <bool FUSED>
+__device__ __forceinline__ long long table_len(const RunDesc& rd) {
+  return (long long)rd.nbuckets * (FUSED ? rd.fchunks : rd.nchunks);
+}
+template <bool FUSED>
+__device__ __forceinline__ long long table_base(const RunDesc& rd) {
+  return FUSED ? rd.ftbase : rd.tbase;
+}
+
 // exclusive scan of one PIC_CELL_CHUNK slice of a run's table, in place
+template <bool FUSED>
 __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restrict__ runs,
                                                         int* __restrict__ table,
                                                         int* __restrict__ chunk_sums,
@@ -400,12 +422,12 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restric
   __shared__ int wsum[4];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
-  const long long len = (long long)rd.nbuckets * rd.nchunks;
+  const long long len = table_len<FUSED>(rd), tb = table_base<FUSED>(rd);
   const long long c0 = it.y + threadIdx.x * 8;
   int v[8], s = 0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    v[k] = (c0 + k < len) ? table[rd.tbase + c0 + k] : 0;
+    v[k] = (c0 + k < len) ? table[tb + c0 + k] : 0;
     s += v[k];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -422,31 +444,50 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restric
   int ex = wpre + inc - s;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (c0 + k < len) table[rd.tbase + c0 + k] = ex;
+    if (c0 + k < len) table[tb + c0 + k] = ex;
     ex += v[k];
   }
   if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
 }
 
+template <bool FUSED>
 __global__ __launch_bounds__(PIC_WG) void k_tscan_chunks(const RunDesc* __restrict__ runs,
                                                          int* __restrict__ chunk_sums) {
   __shared__ int wsum[4];
   const RunDesc rd = runs[blockIdx.x];
-  block_exclusive_scan(chunk_sums + rd.tscan_item_begin,
-                       rd.tscan_item_end - rd.tscan_item_begin, wsum);
+  const int b = FUSED ? rd.fscan_item_begin : rd.tscan_item_begin;
+  const int e = FUSED ? rd.fscan_item_end : rd.tscan_item_end;
+  block_exclusive_scan(chunk_sums + b, e - b, wsum);
 }
 
+template <bool FUSED>
 __global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict__ runs,
                                                       int* __restrict__ table,
                                                       const int* __restrict__ chunk_sums,
                                                       const int2* __restrict__ items) {
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
-  const long long len = (long long)rd.nbuckets * rd.nchunks;
+  const long long len = table_len<FUSED>(rd), tb = table_base<FUSED>(rd);
   const int add = chunk_sums[blockIdx.x];
   for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
     const long long c = (long long)it.y + k;
-    if (c < len) table[rd.tbase + c] += add;
+    if (c < len) table[tb + c] += add;
+  }
+}
+
+// tile offsets after the fused sort: the base of each bucket's first chunk
+__global__ __launch_bounds__(PIC_WG) void k_ftoff(const RunDesc* __restrict__ runs,
+                                                  RunState* __restrict__ st,
+                                                  const int* __restrict__ table,
+                                                  int* __restrict__ toff) {
+  const RunDesc rd = runs[blockIdx.x];
+  if (rd.fchunks <= 0) return;
+  for (int b = threadIdx.x; b < rd.nbuckets; b += PIC_WG)
+    toff[rd.toffbase + b] = table[rd.ftbase + (long long)b * rd.fchunks];
+  if (threadIdx.x == 0) {
+    const int np = st[blockIdx.x].np;
+    toff[rd.toffbase + rd.nbuckets] = np;
+    st[blockIdx.x].nsorted = np;
   }
 }
 
@@ -476,32 +517,49 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
   __syncthreads();
   const long long pb = rd.pbase;
   const int p1 = min(it.y + PIC_CHUNK, np);
-  for (int q = it.y; q < p1; q += 64) {
-    const int p = q + lane;
-    const bool act = p < p1;
-    R vx = 0, vy = 0, vvx = 0, vvy = 0;
-    int vid = 0, key = -1;
-    if (act) {
-      vx = src.x[pb + p]; vy = src.y[pb + p]; vvx = src.vx[pb + p]; vvy = src.vy[pb + p];
-      vid = src.id[pb + p];
-      key = bucket_of<R>(vx, vy, rd);
+  // SCAT_U groups of 64 particles per pass: all their loads are issued first (the wave's
+  // memory latency is paid once per pass), then the groups are ranked in index order
+  constexpr int SCAT_U = 4;
+  for (int q0 = it.y; q0 < p1; q0 += 64 * SCAT_U) {
+    R vx[SCAT_U], vy[SCAT_U], vvx[SCAT_U], vvy[SCAT_U];
+    int vid[SCAT_U], key[SCAT_U];
+#pragma unroll
+    for (int u = 0; u < SCAT_U; ++u) {
+      const int p = q0 + 64 * u + lane;
+      const bool act = p < p1;
+      vx[u] = act ? src.x[pb + p] : (R)0;
+      vy[u] = act ? src.y[pb + p] : (R)0;
+      vvx[u] = act ? src.vx[pb + p] : (R)0;
+      vvy[u] = act ? src.vy[pb + p] : (R)0;
+      vid[u] = act ? src.id[pb + p] : 0;
     }
-    unsigned long long rem = __ballot(act);
-    int d = 0;
-    while (rem) {
-      const int leader = __ffsll((long long)rem) - 1;
-      const int lk = __shfl(key, leader, 64);
-      const unsigned long long m = __ballot(act && key == lk);
-      const int b0 = base[lk];
-      if (act && key == lk) d = b0 + __popcll(m & ((1ull << lane) - 1ull));
-      __syncthreads();
-      if (lane == leader) base[lk] = b0 + __popcll(m);
-      __syncthreads();
-      rem &= ~m;
+#pragma unroll
+    for (int u = 0; u < SCAT_U; ++u) {
+      const int p = q0 + 64 * u + lane;
+      key[u] = (p < p1) ? bucket_of<R>(vx[u], vy[u], rd) : -1;
     }
-    if (act) {
-      dst.x[pb + d] = vx; dst.y[pb + d] = vy; dst.vx[pb + d] = vvx; dst.vy[pb + d] = vvy;
-      dst.id[pb + d] = vid;
+#pragma unroll
+    for (int u = 0; u < SCAT_U; ++u) {
+      const int p = q0 + 64 * u + lane;
+      const bool act = p < p1;
+      unsigned long long rem = __ballot(act);
+      int d = 0;
+      while (rem) {
+        const int leader = __ffsll((long long)rem) - 1;
+        const int lk = __shfl(key[u], leader, 64);
+        const unsigned long long m = __ballot(act && key[u] == lk);
+        const int b0 = base[lk];
+        if (act && key[u] == lk) d = b0 + __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();
+        if (lane == leader) base[lk] = b0 + __popcll(m);
+        __syncthreads();
+        rem &= ~m;
+      }
+      if (act) {
+        dst.x[pb + d] = vx[u]; dst.y[pb + d] = vy[u]; dst.vx[pb + d] = vvx[u];
+        dst.vy[pb + d] = vvy[u];
+        dst.id[pb + d] = vid[u];
+      }
     }
   }
 }
@@ -846,11 +904,22 @@ void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nrun
                       int npitems) {
   const size_t lds = sizeof(int) * (size_t)sb.max_buckets;
   k_tile_hist<R><<<npitems, PIC_WG, lds, s>>>(runs, st, src.x, src.y, sb.table, pitems);
-  k_tscan_local<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
-                                                   sb.tscan_items);
-  k_tscan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
-  k_tscan_add<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums, sb.tscan_items);
+  k_tscan_local<false><<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
+                                                          sb.tscan_items);
+  k_tscan_chunks<false><<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
+  k_tscan_add<false><<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
+                                                        sb.tscan_items);
   k_tile_scatter<R><<<npitems, 64, lds, s>>>(runs, st, src, dst, sb.table, sb.toff, pitems);
+}
+
+void launch_fused_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                       SortBufs sb, int* toff_new) {
+  k_tscan_local<true><<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
+                                                         sb.fscan_items);
+  k_tscan_chunks<true><<<nruns, PIC_WG, 0, s>>>(runs, sb.fchunk_sums);
+  k_tscan_add<true><<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
+                                                       sb.fscan_items);
+  k_ftoff<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.ftable, toff_new);
 }
 
 void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,

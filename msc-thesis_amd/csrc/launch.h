@@ -27,8 +27,13 @@ struct SortBufs {
   int* tchunk_sums;             // [n table-scan items]
   const int2* tscan_items;      // {run, first entry} items of PIC_CELL_CHUNK table entries
   int ntscan_items;
-  int* toff;                    // [sum (nbuckets+1)] tile offsets
+  int* toff;                    // [sum (nbuckets+1)] tile offsets written by the sort
   int max_buckets;
+  // sort fused into the push
+  int* ftable;                  // [sum nbuckets*fchunks] counts, then exclusive bases
+  int* fchunk_sums;             // [n fused table-scan items]
+  const int2* fscan_items;      // {run, first entry} items of PIC_CELL_CHUNK entries
+  int nfscan_items;
 };
 
 template <typename R>
@@ -44,6 +49,9 @@ void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
                  const int2* pitems, int npitems, const int2* citems, int ncitems,
                  long long ctotal, bool move_data);
 // stable tile (bucket) sort of the step; writes sb.toff and RunState::nsorted
+// scan of the fused sort's count table and the new tile offsets (toff_new, nsorted)
+void launch_fused_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                       SortBufs sb, int* toff_new);
 template <typename R>
 void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
                       PartBufs<R> src, PartBufs<R> dst, SortBufs sb, const int2* pitems,
@@ -52,6 +60,9 @@ void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nrun
 void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,
                            SortBufs sb, const int2* pitems, int npitems, const int2* citems,
                            int ncitems, long long ctotal);
+
+#define PUSH_NT 512            // threads per push workgroup
+#define PUSH_NW (PUSH_NT / 64)  // waves per push workgroup
 
 template <typename R>
 struct PushArgs {
@@ -71,6 +82,13 @@ struct PushArgs {
   double* diag;
   int mode;                     // 0: gather+push+deposit, 1: deposit only
   int lds_bytes;
+  // sort fused into the push (mode 0, tile items): 0 off; 1 count the tiles of the new
+  // positions per wave chunk into ftable; 2 scatter the particles tile-major (by their
+  // positions at the start of this step) into dst, with their ids
+  int sort_mode;
+  int* ftable;
+  PartBufs<R> dst;
+  const int* id;                // ids of the source buffer
 };
 template <typename R>
 void launch_push(hipStream_t s, const PushArgs<R>& a);

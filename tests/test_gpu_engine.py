@@ -89,9 +89,26 @@ def run_state(cfg, parts, nsteps, graphs=True, tuning=None):
         if tuning:
             sim.set_tuning(**tuning)
         sim.step(nsteps)
+        sim.sync()  # raises on a device error flag (capacity, sort slots)
         out = sim.particles(sync=False)
         diag = sim.diagnostics(0, nsteps)
     return out, diag
+
+
+def test_single_steps_with_reads_between_match_a_block_of_steps():
+    """The sort fused into the push spans two steps (count, then scatter): stepping one
+    step at a time, with particle reads in between, must give the same state."""
+    cfg, parts = small_landau(n=64, ppc=16)
+    ref, _ = run_state(cfg, parts, 13)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        for _ in range(13):
+            sim.step(1)
+            sim.particles(sync=True)
+        sim.sync()
+        got = sim.particles(sync=False)
+    for u, v in zip(ref, got):
+        assert np.array_equal(u, v)
 
 
 def test_deterministic_and_graph_equivalent():
