@@ -142,6 +142,7 @@ def main():
     elif not args.no_autotune:
         sim.autotune(trial_steps=8)
     tune = sim.tuning()
+    fused = sim.sort_fused()
     sim.step(args.warmup)
     sim.sync()
     torch.cuda.synchronize()
@@ -159,8 +160,12 @@ def main():
 
     G = cfg.nx * cfg.ny
     # push_deposit algorithmic bytes per launch: x,y,vx,vy read + written (32 B/particle),
-    # padded E read (8 B/node) and fixed-point rho written (8 B/node)
+    # padded E read (8 B/node) and fixed-point rho written (8 B/node); with the sort fused
+    # into the push, one launch in sort_every also reads and writes the particle ids
+    # (8 B/particle) — the launch average below spans all three kinds of launch
     alg_bytes = 32.0 * n + 16.0 * G
+    if fused and tune["sort_every"] > 0:
+        alg_bytes += 8.0 * n / tune["sort_every"]
     push_ms = stages["push_deposit"]
     achieved = alg_bytes / (push_ms * 1e-3) / 1e9
     traffic = read_traffic()
@@ -179,7 +184,7 @@ def main():
         "data": "synthetic (seeded Landau quiet-start loader)",
         "config": {"workload": f"landau_{cfg.nx}x{cfg.ny}_{args.ppc}ppc", "grid": [cfg.nx, cfg.ny],
                    "particles": n, "dt": cfg.dt, "field": "f64 FFT Poisson",
-                   "charge": "int64 fixed-point", "tuning": tune},
+                   "charge": "int64 fixed-point", "tuning": tune, "sort_fused": fused},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_push (fused gather+push+deposit)",

@@ -947,6 +947,14 @@ int pic_get_tuning(pic_ctx* ctx, int* tile, int* halo, int* splits, int* sort_ev
   return PIC_OK;
 }
 
+int pic_sort_fused(pic_ctx* ctx, int* fused) {
+  if (!ctx || !fused) return fail(PIC_ERR_ARG, "NULL argument");
+  Ctx& x = ctx->impl;
+  if (x.tune.empty()) return fail(PIC_ERR_STATE, "context not allocated yet");
+  *fused = x.fused ? 1 : 0;
+  return PIC_OK;
+}
+
 int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every) {
   CTX_CHECK(ctx);
   Ctx& x = ctx->impl;

@@ -20,14 +20,11 @@ namespace {
 // Cell numbering of the engine's own sorts: tile-major (tile t = (cx/T)*tiles_y + cy/T,
 // then cx%T, cy%T), so that every tile and every cell is one contiguous range.  The
 // operator pic_op_sort sorts whatever keys it is given (cx*ny+cy from pic_op_cell_keys).
+// Tile sizes are powers of two and cells are clamped to >= 0: shifts and masks.
 __device__ __forceinline__ int engine_cell_key(int cx, int cy, const RunDesc& rd) {
-  const int T = rd.tile;
-  const int t = (cx / T) * rd.tiles_y + cy / T;
-  return t * T * T + (cx % T) * T + (cy % T);
-}
-
-__device__ __forceinline__ int tile_of(int cx, int cy, const RunDesc& rd) {
-  return (cx / rd.tile) * rd.tiles_y + cy / rd.tile;
+  const int sh = __builtin_ctz((unsigned)rd.tile), m = rd.tile - 1;
+  const int t = (cx >> sh) * rd.tiles_y + (cy >> sh);
+  return (((t << sh) + (cx & m)) << sh) + (cy & m);
 }
 
 // In-place exclusive scan of a[0..n) by one 256-thread workgroup (tiles of 256 values
@@ -360,13 +357,18 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict
 //                     with equal buckets get consecutive slots (ballot ranks), so the
 //                     order inside a bucket is the previous order: deterministic and
 //                     stable without any per-cell sort.
+// The bucket of a position is its tile.  Every pass of one sort evaluates this same
+// function, so any deterministic cell choice is consistent: the cell comes from the
+// reciprocal multiply (locate_w), the tile from shifts (tile sizes are powers of two).  No
+// per-particle division: the count and scatter passes run it once per particle.
 template <typename R>
-__device__ __forceinline__ int bucket_of(R x, R y, const RunDesc& rd) {
+__device__ __forceinline__ int bucket_of(R x, R y, const Geo<R>& g, const RunDesc& rd) {
   int cx, cy;
   R f;
-  locate(x, (R)rd.dx, rd.nx, cx, f);
-  locate(y, (R)rd.dy, rd.ny, cy, f);
-  return tile_of(cx, cy, rd);
+  locate_w(x, g.idx, g.nx, cx, f);
+  locate_w(y, g.idy, g.ny, cy, f);
+  const int sh = __builtin_ctz((unsigned)rd.tile);
+  return (cx >> sh) * rd.tiles_y + (cy >> sh);
 }
 
 template <typename R>
@@ -379,6 +381,7 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
   extern __shared__ int h[];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
+  const Geo<R> g = make_geo<R>(rd.nx, rd.ny, rd.lx, rd.ly);
   const int np = st[it.x].np;
   const int nb = rd.nbuckets;
   const int chunk = it.y / PIC_CHUNK;
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (q + u * PIC_WG < p1) atomicAdd(&h[bucket_of<R>(xs[u], ys[u], rd)], 1);
+      if (q + u * PIC_WG < p1) atomicAdd(&h[bucket_of<R>(xs[u], ys[u], g, rd)], 1);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += PIC_WG)
@@ -501,6 +504,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
   extern __shared__ int base[];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
+  const Geo<R> g = make_geo<R>(rd.nx, rd.ny, rd.lx, rd.ly);
   const int np = st[it.x].np;
   const int nb = rd.nbuckets;
   const int chunk = it.y / PIC_CHUNK;
@@ -536,7 +540,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {
       const int p = q0 + 64 * u + lane;
-      key[u] = (p < p1) ? bucket_of<R>(vx[u], vy[u], rd) : -1;
+      key[u] = (p < p1) ? bucket_of<R>(vx[u], vy[u], g, rd) : -1;
     }
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {

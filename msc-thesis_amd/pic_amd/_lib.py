@@ -107,6 +107,7 @@ SIGNATURES = {
     "pic_use_graphs": (_I, [_P, _I]),
     "pic_get_tuning": (_I, [_P] + [ctypes.POINTER(_I)] * 4),
     "pic_set_tuning": (_I, [_P, _I, _I, _I, _I]),
+    "pic_sort_fused": (_I, [_P, ctypes.POINTER(_I)]),
     "pic_autotune": (_I, [_P, _I, _DP]),
     "pic_stage_times": (_I, [_P, _I, _DP]),
     "pic_run_compat": (_I, [_DP, _DP, _DP, _DP, _DP, _DP, _I64, _I, _CFG, _DP]),

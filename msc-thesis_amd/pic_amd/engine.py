@@ -169,6 +169,13 @@ class Simulation:
         check(lib.pic_get_tuning(self._ctx, *(ctypes.byref(a) for a in v)))
         return dict(zip(("tile", "halo", "splits", "sort_every"), (a.value for a in v)))
 
+    def sort_fused(self):
+        """True when the tile sort runs inside the push launches (count on the step before
+        a sort step, scatter while pushing on the sort step)."""
+        v = ctypes.c_int()
+        check(lib.pic_sort_fused(self._ctx, ctypes.byref(v)))
+        return bool(v.value)
+
     def set_tuning(self, tile=0, halo=0, splits=0, sort_every=0):
         check(lib.pic_set_tuning(self._ctx, tile, halo, splits, sort_every))
 

@@ -1,5 +1,6 @@
 """GPU parity of the full step (context API) against the CPU oracle, determinism and
 invariance properties, and the plasma physics the step must reproduce."""
+import ctypes
 import math
 
 import numpy as np
@@ -132,6 +133,23 @@ def test_tiling_and_sort_cadence_do_not_change_physics(tuning):
     got, _ = run_state(cfg, parts, 21, tuning=tuning)
     for u, v in zip(ref, got):
         assert np.array_equal(u, v)
+
+
+def test_sort_runs_fused_only_where_it_can():
+    cfg, parts = small_landau(n=64, ppc=16)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.set_tuning(tile=16, halo=8, sort_every=4)
+        assert sim.sort_fused()
+        sim.set_tuning(tile=16, halo=8, sort_every=1)      # nothing to count ahead of
+        assert not sim.sort_fused()
+    cfg = _lib.default_config(nx=64, ny=64, lx=64.0, ly=64.0, precision=_lib.PIC_FP32, seed=5)
+    _lib.check(_lib.lib.pic_mcc_preset(ctypes.byref(cfg), 1e21, 2.0, 1e15))
+    parts = pic_amd.load_landau(64, 64, 64.0, 64.0, 4, vth=2.0, alpha=0.0, mode=0, seed=5)
+    cfg.capacity = int(parts[0].size * 1.25)
+    with Simulation(cfg) as sim:                            # births need the separate sort
+        sim.set_particles(*parts)
+        assert not sim.sort_fused()
 
 
 def test_batch_equals_separate_runs():
