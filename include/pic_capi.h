@@ -268,7 +268,9 @@ int pic_load_checkpoint(pic_ctx* ctx, int run, const char* path);
  * Differentiable collisionless run (no reference counterpart).  The initial state is
  * x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl), y0 = yl, v0 given; the loss is
  * sum(Ex^2 + Ey^2) dx dy after nsteps.  Reverse mode by hand-written adjoint kernels with
- * checkpoints every `checkpoint_every` steps and recomputation in between. */
+ * particle checkpoints every `checkpoint_every` steps and recomputation in between; the
+ * field of every step is kept (nsteps x (nx+1)(ny+1) x 2 reals), so the recomputation is
+ * particle pushes only. */
 typedef struct pic_adjoint pic_adjoint;
 int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl, int64_t n,
                        int nmodes, const int32_t* mx, const int32_t* my, int nsteps,

@@ -1,5 +1,8 @@
-// adjoint.cpp -- differentiable collisionless run (C ABI half): forward with checkpoints
-// every K steps, reverse sweep by segments with recomputation of the K steps in between.
+// adjoint.cpp -- differentiable collisionless run (C ABI half): forward with particle
+// checkpoints every K steps, reverse sweep by segments with recomputation of the K steps in
+// between.  The forward also keeps the field of every step (nsteps x (nx+1)(ny+1) x 2 reals,
+// far smaller than the particle state), so the recomputation is particle pushes only: no
+// deposit and no field solve.
 // No reference counterpart; the forward step is the reference step order of
 // thesisCode/collisionless/double/src/main.cc:137-172 (deposit, Poisson, E, gather, push).
 #include <cmath>
@@ -27,7 +30,7 @@ struct pic_adjoint {
   void *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
   void* ckpt = nullptr;                       // nseg x 4 x n
   void *trx = nullptr, *try_ = nullptr;       // (K+1) x n
-  void* trE = nullptr;                        // K x padded field
+  void* trE = nullptr;                        // nsteps x padded field (E_0 .. E_{N-1})
   double *lx = nullptr, *ly = nullptr, *lvx = nullptr, *lvy = nullptr;
   double *lE = nullptr, *lrho = nullptr, *rho = nullptr;
   unsigned long long *hi = nullptr, *lo = nullptr;
@@ -96,18 +99,18 @@ template <typename R>
 void record_forward(pic_adjoint& a) {
   R *x = (R*)a.x, *y = (R*)a.y;
   for (int k = 0; k < a.nsteps; ++k) {
-    field(a, a.E);
+    R* Ek = (R*)a.at(a.trE, (size_t)k * a.epad());
+    field(a, Ek);
     R* ck = (k % a.K == 0) ? (R*)a.at(a.ckpt, (size_t)(k / a.K) * 4 * a.n) : nullptr;
-    adj_push<R>(a.s, a.g, x, y, x, y, (R*)a.vx, (R*)a.vy, (const R*)a.E, a.n, kick(k), ck, a.hi,
-                a.lo);
+    adj_push<R>(a.s, a.g, x, y, x, y, (R*)a.vx, (R*)a.vy, Ek, a.n, kick(k), ck, a.hi, a.lo);
   }
   field(a, a.E);
   adj_energy<R>(a.s, a.g, (const R*)a.E, a.part, kEnergyBlocks);
 }
 
 // Reverse sweep for dL = 1 (the caller scales the outputs): seed from E_N, then for each
-// segment restore its checkpoint, recompute its steps into the trajectory buffers and run
-// the reverse steps over them.
+// segment restore its checkpoint, recompute its steps into the trajectory buffers (pushes
+// in the stored fields of the forward) and run the reverse steps over them.
 template <typename R>
 void record_backward(pic_adjoint& a) {
   const size_t n = (size_t)a.n;
@@ -117,21 +120,19 @@ void record_backward(pic_adjoint& a) {
   const double cq = a.qp / (a.g.dx * a.g.dy);
   auto X = [&](int slot) { return (R*)a.at(a.trx, (size_t)slot * n); };
   auto Y = [&](int slot) { return (R*)a.at(a.try_, (size_t)slot * n); };
-  auto Ek = [&](int slot) { return (R*)a.at(a.trE, (size_t)slot * a.epad()); };
+  auto Ek = [&](int k) { return (R*)a.at(a.trE, (size_t)k * a.epad()); };
   for (int seg = a.nseg - 1; seg >= 0; --seg) {
     const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
     adj_restore<R>(a.s, a.g, (const R*)a.at(a.ckpt, (size_t)seg * 4 * n), a.n, X(0), Y(0),
-                   (R*)a.vx, (R*)a.vy, a.hi, a.lo);
+                   (R*)a.vx, (R*)a.vy, nullptr, nullptr);
     for (int k = k0; k < k1; ++k) {
       const int sl = k - k0;
-      field(a, Ek(sl));
-      const bool last = (k + 1 == k1);  // x_{k1} is not needed as charge here
-      adj_push<R>(a.s, a.g, X(sl), Y(sl), X(sl + 1), Y(sl + 1), (R*)a.vx, (R*)a.vy, Ek(sl), a.n,
-                  kick(k), nullptr, last ? nullptr : a.hi, a.lo);
+      adj_push<R>(a.s, a.g, X(sl), Y(sl), X(sl + 1), Y(sl + 1), (R*)a.vx, (R*)a.vy, Ek(k), a.n,
+                  kick(k), nullptr, nullptr, nullptr);
     }
     for (int k = k1; k > k0; --k) {
       const int sl = k - k0;
-      adj_pass<R>(a.s, a.g, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1), Ek(sl - 1),
+      adj_pass<R>(a.s, a.g, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1), Ek(k - 1),
                   kick(k - 1) * a.g.dt * a.g.qm, a.lx, a.ly, a.lvx, a.lvy, a.lE, a.n);
       adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
     }
@@ -247,7 +248,7 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   PIC_HIP(A.getb(&A.ckpt, A.rs * 4 * n * A.nseg));
   PIC_HIP(A.getb(&A.trx, A.rs * n * (A.K + 1)));
   PIC_HIP(A.getb(&A.try_, A.rs * n * (A.K + 1)));
-  PIC_HIP(A.getb(&A.trE, A.rs * A.epad() * A.K));
+  PIC_HIP(A.getb(&A.trE, A.rs * A.epad() * (size_t)nsteps));
   PIC_HIP(A.get(&A.lx, n));
   PIC_HIP(A.get(&A.ly, n));
   PIC_HIP(A.get(&A.lvx, n));

@@ -115,7 +115,8 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_push(AdjGrid g, const R* xi, con
   }
 }
 
-// restore a checkpoint (x | y | vx | vy) into (x, y) and (vx, vy); deposit x's charge
+// restore a checkpoint (x | y | vx | vy) into (x, y) and (vx, vy); deposit x's charge into
+// hi/lo unless hi is NULL
 template <typename R>
 __global__ __launch_bounds__(PIC_WG) void k_adj_restore(AdjGrid g, const R* __restrict__ ck,
                                                         int64_t n, R* __restrict__ x,
@@ -131,10 +132,12 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_restore(AdjGrid g, const R* __re
     y[p] = yr;
     vx[p] = ck[2 * n + p];
     vy[p] = ck[3 * n + p];
-    int cx, cy;
-    long long w[4];
-    deposit_weights<R>(xr, yr, q, cx, cy, w);
-    deposit_global<R>(hi, lo, cx, cy, w, q);
+    if (hi) {
+      int cx, cy;
+      long long w[4];
+      deposit_weights<R>(xr, yr, q, cx, cy, w);
+      deposit_global<R>(hi, lo, cx, cy, w, q);
+    }
   }
 }
 
