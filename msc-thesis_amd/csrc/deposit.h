@@ -26,10 +26,17 @@ __device__ __forceinline__ void deposit_weights(R xr, R yr, const Geo<R>& g, int
   R fx, fy;
   locate_w(xr, g.idx, g.nx, cx, fx);
   locate_w(yr, g.idy, g.ny, cy, fy);
-  // per-axis weights in [0, 2^QB]: 32-bit, products by one 32x32->64 multiply each
+  // per-axis weights in [0, 2^QB]: 32-bit, one 32x32->64 product per node
   const unsigned one = 1u << Fix<R>::QB;
-  const unsigned ix1 = (unsigned)qweight(fx), iy1 = (unsigned)qweight(fy);
-  const unsigned ix0 = one - ix1, iy0 = one - iy1;
+  unsigned ix1 = (unsigned)qweight(fx), iy1 = (unsigned)qweight(fy);
+  unsigned ix0 = one - ix1, iy0 = one - iy1;
+  if (Fix<R>::QB <= 23) {
+    // the weights fit in 24 bits (they are <= 2^QB): with the high byte known clear the
+    // compiler uses the full-rate 24-bit multiplies for the 48-bit products; the masks
+    // change no value
+    ix0 &= 0xFFFFFFu; iy0 &= 0xFFFFFFu;
+    ix1 &= 0xFFFFFFu; iy1 &= 0xFFFFFFu;
+  }
   w[0] = (long long)((unsigned long long)ix0 * iy0);
   w[1] = (long long)((unsigned long long)ix1 * iy0);
   w[2] = (long long)((unsigned long long)ix0 * iy1);

@@ -621,8 +621,8 @@ __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* 
   typedef typename EWin<R>::R2 R2;
   R2 e00, e01, e10, e11;
   int rx = cx - w.x0, ry = cy - w.y0;
-  rx += (rx < 0) ? g.nx : ((rx >= g.nx) ? -g.nx : 0);
-  ry += (ry < 0) ? g.ny : ((ry >= g.ny) ? -g.ny : 0);
+  rx = pwrap(rx, g.nx);
+  ry = pwrap(ry, g.ny);
   if (w.ew && rx < w.W - 1 && ry < w.W - 1) {
     const int i0 = rx * w.W + ry;
     e00 = w.ew[i0]; e01 = w.ew[i0 + 1]; e10 = w.ew[i0 + w.W]; e11 = w.ew[i0 + w.W + 1];
@@ -632,21 +632,21 @@ __device__ __forceinline__ void push_one(R& xr, R& yr, R& vxr, R& vyr, const R* 
     e00 = e[i00]; e01 = e[i00 + 1]; e10 = e[i00 + g.ny + 1]; e11 = e[i00 + g.ny + 2];
   }
   R epx = g00 * e00.x;
-  epx = epx + g10 * e10.x;
-  epx = epx + g01 * e01.x;
-  epx = epx + g11 * e11.x;
+  epx = fmar(g10, e10.x, epx);
+  epx = fmar(g01, e01.x, epx);
+  epx = fmar(g11, e11.x, epx);
   R epy = g00 * e00.y;
-  epy = epy + g10 * e10.y;
-  epy = epy + g01 * e01.y;
-  epy = epy + g11 * e11.y;
+  epy = fmar(g10, e10.y, epy);
+  epy = fmar(g01, e01.y, epy);
+  epy = fmar(g11, e11.y, epy);
   const double vsx = (double)vxr + skc * (double)epx, vsy = (double)vyr + skc * (double)epy;
   ke += vsx * vsx + vsy * vsy;
   px += vsx;
   py += vsy;
-  vxr = vxr + hk * epx;
-  vyr = vyr + hk * epy;
-  xr = wrap_pos((R)(xr + dtr * vxr), g.Lx);
-  yr = wrap_pos((R)(yr + dtr * vyr), g.Ly);
+  vxr = fmar(hk, epx, vxr);
+  vyr = fmar(hk, epy, vyr);
+  xr = wrap_pos(fmar(dtr, vxr, xr), g.Lx);
+  yr = wrap_pos(fmar(dtr, vyr, yr), g.Ly);
 }
 
 #include "push_kernel.inc"
@@ -767,19 +767,19 @@ __global__ __launch_bounds__(PIC_WG) void k_op_gather_push(R* x, R* y, R* vx, R*
     const long long n00 = (long long)cx * ny + cy, n10 = (long long)cxp * ny + cy,
                     n01 = (long long)cx * ny + cyp, n11 = (long long)cxp * ny + cyp;
     R epx = g00 * (R)ex[n00];
-    epx = epx + g10 * (R)ex[n10];
-    epx = epx + g01 * (R)ex[n01];
-    epx = epx + g11 * (R)ex[n11];
+    epx = fmar(g10, (R)ex[n10], epx);
+    epx = fmar(g01, (R)ex[n01], epx);
+    epx = fmar(g11, (R)ex[n11], epx);
     R epy = g00 * (R)ey[n00];
-    epy = epy + g10 * (R)ey[n10];
-    epy = epy + g01 * (R)ey[n01];
-    epy = epy + g11 * (R)ey[n11];
+    epy = fmar(g10, (R)ey[n10], epy);
+    epy = fmar(g01, (R)ey[n01], epy);
+    epy = fmar(g11, (R)ey[n11], epy);
     const double vsx = (double)vxr + skc * (double)epx, vsy = (double)vyr + skc * (double)epy;
     ke += vsx * vsx + vsy * vsy;
-    vxr = vxr + hk * epx;
-    vyr = vyr + hk * epy;
-    x[p] = wrap_pos((R)(xr + dtr * vxr), g.Lx);
-    y[p] = wrap_pos((R)(yr + dtr * vyr), g.Ly);
+    vxr = fmar(hk, epx, vxr);
+    vyr = fmar(hk, epy, vyr);
+    x[p] = wrap_pos(fmar(dtr, vxr, xr), g.Lx);
+    y[p] = wrap_pos(fmar(dtr, vyr, yr), g.Ly);
     vx[p] = vxr;
     vy[p] = vyr;
   }

@@ -96,6 +96,18 @@ __device__ __forceinline__ void locate_w(R x, R inv, int n, int& c, R& f) {
   f = fr;
 }
 
+// fused multiply-add with one rounding (v_fma_f32 / v_fma_f64); the particle arithmetic
+// spells out every fma it uses (FP contraction is off), the CPU mirror uses std::fma
+template <typename R>
+__device__ __forceinline__ R fmar(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
+// periodic index on a power-of-two axis (every grid size is a power of two): v mod n for
+// any int v, one AND
+__device__ __forceinline__ int pwrap(int v, int n) { return v & (n - 1); }
+
 template <typename R>
 __device__ __forceinline__ R wrap_pos(R x, R L) {
   if (x >= L) x -= L;

@@ -287,7 +287,8 @@ void push_cont(int64_t n, double* x, double* y, double* vx, double* vy, const do
   if (py) *py = sy;
 }
 
-// GPU-mirror push: per-axis fixed-point gather weights, fixed evaluation order, no FMA.
+// GPU-mirror push: per-axis fixed-point gather weights, fixed evaluation order, the same
+// fused multiply-adds as the device code (std::fma: one rounding, as v_fma_f32/f64).
 template <typename R, int QB>
 void push_mirror(int64_t n, R* x, R* y, R* vx, R* vy, const R* ex, const R* ey, int nx,
                  int ny, double lx, double ly, double dt, double qm, double c, double* ke,
@@ -311,22 +312,22 @@ void push_mirror(int64_t n, R* x, R* y, R* vx, R* vy, const R* ex, const R* ey, 
     size_t n00 = (size_t)cx * ny + cy, n10 = (size_t)cxp * ny + cy, n01 = (size_t)cx * ny + cyp,
            n11 = (size_t)cxp * ny + cyp;
     R epx = g00 * ex[n00];
-    epx = epx + g10 * ex[n10];
-    epx = epx + g01 * ex[n01];
-    epx = epx + g11 * ex[n11];
+    epx = std::fma(g10, ex[n10], epx);
+    epx = std::fma(g01, ex[n01], epx);
+    epx = std::fma(g11, ex[n11], epx);
     R epy = g00 * ey[n00];
-    epy = epy + g10 * ey[n10];
-    epy = epy + g01 * ey[n01];
-    epy = epy + g11 * ey[n11];
+    epy = std::fma(g10, ey[n10], epy);
+    epy = std::fma(g01, ey[n01], epy);
+    epy = std::fma(g11, ey[n11], epy);
     double vsx = (double)vx[p] + sk_c * (double)epx, vsy = (double)vy[p] + sk_c * (double)epy;
     sk += vsx * vsx + vsy * vsy;
     sx += vsx;
     sy += vsy;
-    R nvx = vx[p] + hk * epx, nvy = vy[p] + hk * epy;
+    R nvx = std::fma(hk, epx, vx[p]), nvy = std::fma(hk, epy, vy[p]);
     vx[p] = nvx;
     vy[p] = nvy;
-    x[p] = wrap_pos((R)(x[p] + dtr * nvx), Lx);
-    y[p] = wrap_pos((R)(y[p] + dtr * nvy), Ly);
+    x[p] = wrap_pos(std::fma(dtr, nvx, x[p]), Lx);
+    y[p] = wrap_pos(std::fma(dtr, nvy, y[p]), Ly);
   }
   if (ke) *ke = sk;
   if (px) *px = sx;
@@ -693,15 +694,15 @@ void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy, int nx, int
         size_t n00 = (size_t)cx * ny + cy, n10 = (size_t)cxp * ny + cy,
                n01 = (size_t)cx * ny + cyp, n11 = (size_t)cxp * ny + cyp;
         float epx = g00 * exf[n00];
-        epx = epx + g10 * exf[n10];
-        epx = epx + g01 * exf[n01];
-        epx = epx + g11 * exf[n11];
+        epx = std::fma(g10, exf[n10], epx);
+        epx = std::fma(g01, exf[n01], epx);
+        epx = std::fma(g11, exf[n11], epx);
         float epy = g00 * eyf[n00];
-        epy = epy + g10 * eyf[n10];
-        epy = epy + g01 * eyf[n01];
-        epy = epy + g11 * eyf[n11];
-        vx[p] = vx[p] + hk * epx;
-        vy[p] = vy[p] + hk * epy;
+        epy = std::fma(g10, eyf[n10], epy);
+        epy = std::fma(g01, eyf[n01], epy);
+        epy = std::fma(g11, eyf[n11], epy);
+        vx[p] = std::fma(hk, epx, vx[p]);
+        vy[p] = std::fma(hk, epy, vy[p]);
       }
       break;
     }
