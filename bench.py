@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--only", choices=["batch", "adjoint"], default=None,
+                    help="run only the ensemble comparison or only the adjoint timing "
+                         "(for profiling those legs)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="fixed tile,halo,splits,sort_every instead of the autotuner")
@@ -132,6 +135,15 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}))
+        return
+    if args.only == "batch":
+        print(json.dumps({"batched_vs_sequential": batch_ratio()}))
+        return
+    if args.only == "adjoint":
+        from pic_amd import _lib
+        from pic_amd.adjoint import time_adjoint
+        print(json.dumps({"adjoint_over_forward": {"f32": time_adjoint(precision=_lib.PIC_FP32),
+                                                   "f64": time_adjoint(precision=_lib.PIC_FP64)}}))
         return
     cfg, parts, _ = presets.landau_512(ppc=args.ppc, n=args.grid)
     n = parts[0].size
