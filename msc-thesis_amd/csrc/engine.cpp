@@ -189,8 +189,8 @@ int Ctx::build_items() {
     d.ftbase = fsum;
     fsum += (long long)d.nbuckets * d.fchunks;
     d.fscan_item_begin = (int)fscan_items.size();
-    for (long long s = 0; s < (long long)d.nbuckets * d.fchunks; s += PIC_CELL_CHUNK)
-      fscan_items.push_back(make_int2(r, (int)s));
+    for (int g = 0; (long long)g * FSCAN_ROWS < d.fchunks; ++g)
+      fscan_items.push_back(make_int2(r, g));  // groups of FSCAN_ROWS table rows
     d.fscan_item_end = (int)fscan_items.size();
   }
   if (fsum >= (1ll << 31)) fused = false;
@@ -218,7 +218,7 @@ int Ctx::build_items() {
   if (sb.ftable) (void)hipFree(sb.ftable);
   PIC_HIP(dalloc(&sb.ftable, fused ? fsum : 1));
   if (sb.fchunk_sums) (void)hipFree(sb.fchunk_sums);
-  PIC_HIP(dalloc(&sb.fchunk_sums, fscan_items.size()));
+  PIC_HIP(dalloc(&sb.fchunk_sums, std::max<size_t>(1, fscan_items.size() * (size_t)maxb)));
   sb.segitems = d_segitems;
   sb.nsegitems = (int)segitems.size();
   sb.tscan_items = d_tscan_items;

@@ -406,18 +406,7 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
     table[rd.tbase + (long long)b * rd.nchunks + chunk] = h[b];
 }
 
-// table of a run: the tile sort's (FUSED = false) or the fused sort's
-template <bool FUSED>
-__device__ __forceinline__ long long table_len(const RunDesc& rd) {
-  return (long long)rd.nbuckets * (FUSED ? rd.fchunks : rd.nchunks);
-}
-template <bool FUSED>
-__device__ __forceinline__ long long table_base(const RunDesc& rd) {
-  return FUSED ? rd.ftbase : rd.tbase;
-}
-
-// exclusive scan of one PIC_CELL_CHUNK slice of a run's table, in place
-template <bool FUSED>
+// exclusive scan of one PIC_CELL_CHUNK slice of a run's tile table, in place
 __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restrict__ runs,
                                                         int* __restrict__ table,
                                                         int* __restrict__ chunk_sums,
@@ -425,7 +414,7 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restric
   __shared__ int wsum[4];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
-  const long long len = table_len<FUSED>(rd), tb = table_base<FUSED>(rd);
+  const long long len = (long long)rd.nbuckets * rd.nchunks, tb = rd.tbase;
   const long long c0 = it.y + threadIdx.x * 8;
   int v[8], s = 0;
 #pragma unroll
@@ -453,24 +442,21 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_local(const RunDesc* __restric
   if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
 }
 
-template <bool FUSED>
 __global__ __launch_bounds__(PIC_WG) void k_tscan_chunks(const RunDesc* __restrict__ runs,
                                                          int* __restrict__ chunk_sums) {
   __shared__ int wsum[4];
   const RunDesc rd = runs[blockIdx.x];
-  const int b = FUSED ? rd.fscan_item_begin : rd.tscan_item_begin;
-  const int e = FUSED ? rd.fscan_item_end : rd.tscan_item_end;
-  block_exclusive_scan(chunk_sums + b, e - b, wsum);
+  block_exclusive_scan(chunk_sums + rd.tscan_item_begin,
+                       rd.tscan_item_end - rd.tscan_item_begin, wsum);
 }
 
-template <bool FUSED>
 __global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict__ runs,
                                                       int* __restrict__ table,
                                                       const int* __restrict__ chunk_sums,
                                                       const int2* __restrict__ items) {
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
-  const long long len = table_len<FUSED>(rd), tb = table_base<FUSED>(rd);
+  const long long len = (long long)rd.nbuckets * rd.nchunks, tb = rd.tbase;
   const int add = chunk_sums[blockIdx.x];
   for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
     const long long c = (long long)it.y + k;
@@ -478,19 +464,111 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict_
   }
 }
 
-// tile offsets after the fused sort: the base of each bucket's first chunk
-__global__ __launch_bounds__(PIC_WG) void k_ftoff(const RunDesc* __restrict__ runs,
-                                                  RunState* __restrict__ st,
-                                                  const int* __restrict__ table,
-                                                  int* __restrict__ toff) {
+// ---- scan of the fused sort's count table.  Row-major: row c (one wave of the push grid)
+// holds that wave's count for every tile, so each wave reads and writes one contiguous
+// row.  The destination slot of (tile b, row c) is
+//   base[b] + sum_{c' < c} count[c'][b],   base[b] = sum_{b' < b} sum_c count[c][b'],
+// i.e. tile-major order with rows in item order.  Rows are scanned in groups of
+// FSCAN_ROWS: k_fscan_part sums each group's columns, k_fscan_base scans the group sums
+// down each column and the column totals across tiles (one workgroup per run; it also
+// writes the tile offsets of the new order), k_fscan_rows writes the slots.
+
+// FSCAN_UNROLL independent loads per thread are issued before any of them is used
+#define FSCAN_UNROLL 32
+
+__global__ __launch_bounds__(PIC_WG) void k_fscan_part(const RunDesc* __restrict__ runs,
+                                                       const int* __restrict__ table,
+                                                       int* __restrict__ part, int stride,
+                                                       const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int nb = rd.nbuckets;
+  const int r0 = it.y * FSCAN_ROWS, r1 = min(r0 + FSCAN_ROWS, rd.fchunks);
+  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
+    const int* col = table + rd.ftbase + b;
+    int sum = 0;
+    for (int c0 = r0; c0 < r1; c0 += FSCAN_UNROLL) {
+      int v[FSCAN_UNROLL];
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k)
+        v[k] = (c0 + k < r1) ? col[(long long)(c0 + k) * nb] : 0;
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k) sum += v[k];
+    }
+    part[(long long)blockIdx.x * stride + b] = sum;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_fscan_base(const RunDesc* __restrict__ runs,
+                                                       RunState* __restrict__ st,
+                                                       int* __restrict__ part, int stride,
+                                                       int* __restrict__ toff) {
+  __shared__ int tot[512];  // one per tile (the fused sort allows at most 512)
+  __shared__ int wsum[4];
   const RunDesc rd = runs[blockIdx.x];
-  if (rd.fchunks <= 0) return;
-  for (int b = threadIdx.x; b < rd.nbuckets; b += PIC_WG)
-    toff[rd.toffbase + b] = table[rd.ftbase + (long long)b * rd.fchunks];
+  if (rd.fchunks <= 0) return;  // uniform: this run sorts with the separate kernels
+  const int nb = rd.nbuckets, g0 = rd.fscan_item_begin, g1 = rd.fscan_item_end;
+  // column totals over the row groups
+  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
+    int sum = 0;
+    for (int gi = g0; gi < g1; gi += FSCAN_UNROLL) {
+      int v[FSCAN_UNROLL];
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k)
+        v[k] = (gi + k < g1) ? part[(long long)(gi + k) * stride + b] : 0;
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k) sum += v[k];
+    }
+    tot[b] = sum;
+  }
+  __syncthreads();
+  block_exclusive_scan(tot, nb, wsum);
+  // each group's first slot per tile: the tile base plus the groups before it
+  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
+    int run = tot[b];
+    toff[rd.toffbase + b] = run;
+    for (int gi = g0; gi < g1; gi += FSCAN_UNROLL) {
+      int v[FSCAN_UNROLL];
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k)
+        v[k] = (gi + k < g1) ? part[(long long)(gi + k) * stride + b] : 0;
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k) {
+        if (gi + k < g1) part[(long long)(gi + k) * stride + b] = run;
+        run += v[k];
+      }
+    }
+  }
   if (threadIdx.x == 0) {
     const int np = st[blockIdx.x].np;
-    toff[rd.toffbase + rd.nbuckets] = np;
+    toff[rd.toffbase + nb] = np;
     st[blockIdx.x].nsorted = np;
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_fscan_rows(const RunDesc* __restrict__ runs,
+                                                       int* __restrict__ table,
+                                                       const int* __restrict__ part,
+                                                       int stride,
+                                                       const int2* __restrict__ items) {
+  const int2 it = items[blockIdx.x];
+  const RunDesc rd = runs[it.x];
+  const int nb = rd.nbuckets;
+  const int r0 = it.y * FSCAN_ROWS, r1 = min(r0 + FSCAN_ROWS, rd.fchunks);
+  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
+    int* col = table + rd.ftbase + b;
+    int run = part[(long long)blockIdx.x * stride + b];
+    for (int c0 = r0; c0 < r1; c0 += FSCAN_UNROLL) {
+      int v[FSCAN_UNROLL];
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k)
+        v[k] = (c0 + k < r1) ? col[(long long)(c0 + k) * nb] : 0;
+#pragma unroll
+      for (int k = 0; k < FSCAN_UNROLL; ++k) {
+        if (c0 + k < r1) col[(long long)(c0 + k) * nb] = run;
+        run += v[k];
+      }
+    }
   }
 }
 
@@ -908,22 +986,21 @@ void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nrun
                       int npitems) {
   const size_t lds = sizeof(int) * (size_t)sb.max_buckets;
   k_tile_hist<R><<<npitems, PIC_WG, lds, s>>>(runs, st, src.x, src.y, sb.table, pitems);
-  k_tscan_local<false><<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
-                                                          sb.tscan_items);
-  k_tscan_chunks<false><<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
-  k_tscan_add<false><<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
-                                                        sb.tscan_items);
+  k_tscan_local<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
+                                                   sb.tscan_items);
+  k_tscan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
+  k_tscan_add<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
+                                                 sb.tscan_items);
   k_tile_scatter<R><<<npitems, 64, lds, s>>>(runs, st, src, dst, sb.table, sb.toff, pitems);
 }
 
 void launch_fused_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
                        SortBufs sb, int* toff_new) {
-  k_tscan_local<true><<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
-                                                         sb.fscan_items);
-  k_tscan_chunks<true><<<nruns, PIC_WG, 0, s>>>(runs, sb.fchunk_sums);
-  k_tscan_add<true><<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
-                                                       sb.fscan_items);
-  k_ftoff<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.ftable, toff_new);
+  k_fscan_part<<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
+                                                  sb.max_buckets, sb.fscan_items);
+  k_fscan_base<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.fchunk_sums, sb.max_buckets, toff_new);
+  k_fscan_rows<<<sb.nfscan_items, PIC_WG, 0, s>>>(runs, sb.ftable, sb.fchunk_sums,
+                                                  sb.max_buckets, sb.fscan_items);
 }
 
 void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, const int* keys,

@@ -30,9 +30,9 @@ struct SortBufs {
   int* toff;                    // [sum (nbuckets+1)] tile offsets written by the sort
   int max_buckets;
   // sort fused into the push
-  int* ftable;                  // [sum nbuckets*fchunks] counts, then exclusive bases
-  int* fchunk_sums;             // [n fused table-scan items]
-  const int2* fscan_items;      // {run, first entry} items of PIC_CELL_CHUNK entries
+  int* ftable;                  // [sum fchunks*nbuckets] row-major counts, then slots
+  int* fchunk_sums;             // [n fused table-scan items][max_buckets] group sums
+  const int2* fscan_items;      // {run, row group} items of FSCAN_ROWS table rows
   int nfscan_items;
 };
 
@@ -61,6 +61,7 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
                            SortBufs sb, const int2* pitems, int npitems, const int2* citems,
                            int ncitems, long long ctotal);
 
+#define FSCAN_ROWS 64  // rows (waves of the push grid) per fused count-table scan item
 #define PUSH_NT 512            // threads per push workgroup
 #define PUSH_NW (PUSH_NT / 64)  // waves per push workgroup
 
