@@ -2,7 +2,10 @@
 // checkpoints every K steps, reverse sweep by segments with recomputation of the K steps in
 // between.  The forward also keeps the field of every step (nsteps x (nx+1)(ny+1) x 2 reals,
 // far smaller than the particle state), so the recomputation is particle pushes only: no
-// deposit and no field solve.
+// deposit and no field solve.  At every checkpoint the particles are counting-sorted by
+// tile (the checkpoint holds the sorted state and the original index of every slot), so the
+// deposits of the segment's forward pushes and reverse passes sum in LDS windows; the
+// per-particle adjoint arrays follow the order of the segment being reversed.
 // No reference counterpart; the forward step is the reference step order of
 // thesisCode/collisionless/double/src/main.cc:137-172 (deposit, Poisson, E, gather, push).
 #include <cmath>
@@ -40,6 +43,16 @@ struct pic_adjoint {
   RunState* st = nullptr;
   int2 *rp = nullptr, *cl = nullptr, *rw = nullptr;
   int nrp = 0, ncl = 0, nrw = 0;
+  // tile sort at the checkpoints and windowed kernels
+  int nb = 0;                                 // tiles
+  int* ids = nullptr;                         // nseg x n: original index of each slot
+  int* iota = nullptr;                        // n: the original order
+  int* toffs = nullptr;                       // nseg x (nb + 1) tile offsets
+  int4* items = nullptr;                      // {tile x, tile y, split} push / pass items
+  int2* pitems = nullptr;                     // sort chunk items
+  int npitems = 0;
+  SortBufs sb;
+  double* tmp = nullptr;                      // 4 x n: adjoint arrays in original order
   int row_lds = 0;  // complex values per row-pass workgroup
   double* part = nullptr;
   bool forward_done = false;
@@ -97,12 +110,26 @@ inline double kick(int k) { return k == 0 ? 0.5 : 1.0; }
 // saving (x, y, vx, vy) every K steps; then E_N and its energy partials.
 template <typename R>
 void record_forward(pic_adjoint& a) {
-  R *x = (R*)a.x, *y = (R*)a.y;
+  R *x = (R*)a.x, *y = (R*)a.y, *vx = (R*)a.vx, *vy = (R*)a.vy;
+  const size_t n = (size_t)a.n;
   for (int k = 0; k < a.nsteps; ++k) {
     R* Ek = (R*)a.at(a.trE, (size_t)k * a.epad());
     field(a, Ek);
-    R* ck = (k % a.K == 0) ? (R*)a.at(a.ckpt, (size_t)(k / a.K) * 4 * a.n) : nullptr;
-    adj_push<R>(a.s, a.g, x, y, x, y, (R*)a.vx, (R*)a.vy, Ek, a.n, kick(k), ck, a.hi, a.lo);
+    const int seg = k / a.K;
+    const int* toff = a.toffs + (size_t)seg * (a.nb + 1);
+    if (k % a.K == 0) {
+      // checkpoint = the state sorted by tile, with the original index of every slot
+      R* ck = (R*)a.at(a.ckpt, (size_t)seg * 4 * n);
+      PartBufs<R> src{x, y, vx, vy, seg == 0 ? a.iota : a.ids + (size_t)(seg - 1) * n};
+      PartBufs<R> dst{ck, ck + n, ck + 2 * n, ck + 3 * n, a.ids + (size_t)seg * n};
+      SortBufs sb = a.sb;
+      sb.toff = a.toffs + (size_t)seg * (a.nb + 1);
+      launch_tile_sort<R>(a.s, a.dd, a.st, 1, src, dst, sb, a.pitems, a.npitems);
+      R* cur[4] = {x, y, vx, vy};
+      for (int q = 0; q < 4; ++q)
+        (void)hipMemcpyAsync(cur[q], ck + q * n, a.rs * n, hipMemcpyDeviceToDevice, a.s);
+    }
+    adj_push_win<R>(a.s, a.g, a.items, toff, x, y, vx, vy, Ek, kick(k), a.hi, a.lo);
   }
   field(a, a.E);
   adj_energy<R>(a.s, a.g, (const R*)a.E, a.part, kEnergyBlocks);
@@ -123,6 +150,12 @@ void record_backward(pic_adjoint& a) {
   auto Ek = [&](int k) { return (R*)a.at(a.trE, (size_t)k * a.epad()); };
   for (int seg = a.nseg - 1; seg >= 0; --seg) {
     const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
+    const int* toff = a.toffs + (size_t)seg * (a.nb + 1);
+    if (seg < a.nseg - 1) {
+      // adjoint arrays from the order of segment seg + 1 to the order of segment seg
+      adj_to_orig(a.s, a.ids + (size_t)(seg + 1) * n, a.lx, a.ly, a.lvx, a.lvy, a.tmp, a.n);
+      adj_from_orig(a.s, a.ids + (size_t)seg * n, a.tmp, a.lx, a.ly, a.lvx, a.lvy, a.n);
+    }
     adj_restore<R>(a.s, a.g, (const R*)a.at(a.ckpt, (size_t)seg * 4 * n), a.n, X(0), Y(0),
                    (R*)a.vx, (R*)a.vy, nullptr, nullptr);
     for (int k = k0; k < k1; ++k) {
@@ -132,14 +165,17 @@ void record_backward(pic_adjoint& a) {
     }
     for (int k = k1; k > k0; --k) {
       const int sl = k - k0;
-      adj_pass<R>(a.s, a.g, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1), Ek(k - 1),
-                  kick(k - 1) * a.g.dt * a.g.qm, a.lx, a.ly, a.lvx, a.lvy, a.lE, a.n);
+      adj_pass_win<R>(a.s, a.g, a.items, toff, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1),
+                      Ek(k - 1), kick(k - 1) * a.g.dt * a.g.qm, a.lx, a.ly, a.lvx, a.lvy,
+                      a.lE);
       adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
     }
   }
   // deposit Jacobian of step 0 at x_0 (slot 0 of segment 0)
   adj_pass<R>(a.s, a.g, X(0), Y(0), a.lrho, cq, nullptr, nullptr, nullptr, 0.0, a.lx, a.ly,
               a.lvx, a.lvy, a.lE, a.n);
+  // results in the original particle order: tmp = (lx, ly, lvx, lvy)
+  adj_to_orig(a.s, a.ids, a.lx, a.ly, a.lvx, a.lvy, a.tmp, a.n);
 }
 
 template <typename F>
@@ -187,11 +223,14 @@ int backward_t(pic_adjoint& a, double dloss, double* grad_amps, void* gvx, void*
     if (rc) return rc;
   }
   PIC_HIP(hipGraphLaunch(a.gbwd, a.s));
+  const double* lx0 = a.tmp;                 // original order
+  const double* lvx0 = a.tmp + 2 * (size_t)a.n;
+  const double* lvy0 = a.tmp + 3 * (size_t)a.n;
   if (grad_amps)
-    adj_amp_grad(a.s, a.lx, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, kEnergyBlocks, dloss,
+    adj_amp_grad(a.s, lx0, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, kEnergyBlocks, dloss,
                  grad_amps);
-  if (gvx) adj_to_r<R>(a.s, a.lvx, dloss, (R*)gvx, a.n);
-  if (gvy) adj_to_r<R>(a.s, a.lvy, dloss, (R*)gvy, a.n);
+  if (gvx) adj_to_r<R>(a.s, lvx0, dloss, (R*)gvx, a.n);
+  if (gvy) adj_to_r<R>(a.s, lvy0, dloss, (R*)gvy, a.n);
   PIC_HIP(hipGetLastError());
   PIC_HIP(hipStreamSynchronize(a.s));
   return PIC_OK;
@@ -204,7 +243,8 @@ extern "C" {
 int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl, int64_t n,
                        int nmodes, const int32_t* mx, const int32_t* my, int nsteps,
                        int checkpoint_every, pic_adjoint** out) {
-  if (!cfg || !xl || !yl || !out || n < 1 || nmodes < 0 || nmodes > 64 || nsteps < 1 ||
+  if (!cfg || !xl || !yl || !out || n < 1 || n > 0x7fffffff || nmodes < 0 || nmodes > 64 ||
+      nsteps < 1 ||
       checkpoint_every < 1 || (nmodes > 0 && (!mx || !my)))
     return fail(PIC_ERR_ARG, "bad arguments to pic_adjoint_create");
   int rc = validate_config(*cfg);
@@ -291,8 +331,61 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   fft_groups(g.nx, g.nx, g.ny, &d.fft_pp, &d.fft_rf);
   A.row_lds = std::max(d.fft_pp, d.fft_rf) * g.ny;
   d.row_item_begin = 0; d.row_item_end = g.nx / d.fft_rf;
+  // Tiles of the windowed kernels: about an eighth of the grid side (4 .. 32 cells), a halo
+  // of up to 16 cells (the drift of a segment), a window that never wraps onto itself, and
+  // about 2048 particles per item.
+  {
+    const int m = std::min(g.nx, g.ny);
+    int T = 4;
+    while (T * 2 <= std::min(32, m / 8)) T *= 2;
+    T = std::max(1, std::min(T, m / 2));
+    g.tile = T;
+    g.halo = std::max(0, std::min(16, (m - T - 1) / 2));
+    g.win = T + 2 * g.halo + 1;
+    g.tiles_y = g.ny / T;
+    A.nb = (g.nx / T) * g.tiles_y;
+    g.splits = (int)std::max<int64_t>(1, std::min<int64_t>(64, (n + 2048LL * A.nb - 1) /
+                                                                  (2048LL * A.nb)));
+    g.nitems = A.nb * g.splits;
+    d.tile = T; d.halo = g.halo; d.tiles_x = g.nx / T; d.tiles_y = g.tiles_y;
+    d.win = g.win; d.splits = g.splits; d.nbuckets = A.nb;
+    d.nchunks = (int)((n + PIC_CHUNK - 1) / PIC_CHUNK);
+    d.pcap = (int)n; d.tbase = 0; d.toffbase = 0; d.pbase = 0;
+    std::vector<int2> ts, pi;
+    for (long long s = 0; s < (long long)A.nb * d.nchunks; s += PIC_CELL_CHUNK)
+      ts.push_back(make_int2(0, (int)s));
+    d.tscan_item_begin = 0; d.tscan_item_end = (int)ts.size();
+    for (int64_t s = 0; s < n; s += PIC_CHUNK) pi.push_back(make_int2(0, (int)s));
+    std::vector<int4> it;
+    for (int tx = 0; tx < g.nx / T; ++tx)
+      for (int ty = 0; ty < g.tiles_y; ++ty)
+        for (int sp = 0; sp < g.splits; ++sp) it.push_back(make_int4(tx, ty, sp, 0));
+    int2* dts = nullptr;
+    PIC_HIP(A.get(&dts, ts.size()));
+    PIC_HIP(hipMemcpy(dts, ts.data(), sizeof(int2) * ts.size(), hipMemcpyHostToDevice));
+    PIC_HIP(A.get(&A.pitems, pi.size()));
+    PIC_HIP(hipMemcpy(A.pitems, pi.data(), sizeof(int2) * pi.size(), hipMemcpyHostToDevice));
+    A.npitems = (int)pi.size();
+    PIC_HIP(A.get(&A.items, it.size()));
+    PIC_HIP(hipMemcpy(A.items, it.data(), sizeof(int4) * it.size(), hipMemcpyHostToDevice));
+    std::memset(&A.sb, 0, sizeof(A.sb));
+    PIC_HIP(A.get(&A.sb.table, (size_t)A.nb * d.nchunks));
+    PIC_HIP(A.get(&A.sb.tchunk_sums, ts.size()));
+    A.sb.tscan_items = dts;
+    A.sb.ntscan_items = (int)ts.size();
+    A.sb.max_buckets = A.nb;
+    PIC_HIP(A.get(&A.ids, (size_t)A.nseg * n));
+    PIC_HIP(A.get(&A.iota, n));
+    PIC_HIP(A.get(&A.toffs, (size_t)A.nseg * (A.nb + 1)));
+    PIC_HIP(A.get(&A.tmp, 4 * (size_t)n));
+    adj_iota(A.s, A.iota, n);
+    PIC_HIP(hipStreamSynchronize(A.s));
+  }
   PIC_HIP(hipMemcpy(A.dd, &d, sizeof(d), hipMemcpyHostToDevice));
-  PIC_HIP(hipMemset(A.st, 0, sizeof(RunState)));
+  RunState st0;
+  std::memset(&st0, 0, sizeof(st0));
+  st0.np = (int)n;
+  PIC_HIP(hipMemcpy(A.st, &st0, sizeof(RunState), hipMemcpyHostToDevice));
   PIC_HIP(hipMemset(A.hi, 0, sizeof(unsigned long long) * G));
   PIC_HIP(hipMemset(A.lo, 0, sizeof(unsigned long long) * G));
   std::vector<int2> rp, cl, rw;

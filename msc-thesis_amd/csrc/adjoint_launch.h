@@ -6,6 +6,8 @@
 struct AdjGrid {
   int nx, ny, lognx, logny;
   double lx, ly, dx, dy, dt, qm;
+  // tiling of the windowed kernels (particles tile-sorted at every checkpoint)
+  int tile, halo, win, tiles_y, splits, nitems;
 };
 
 template <typename R>
@@ -26,6 +28,25 @@ template <typename R>
 void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const double* lrho,
               double cq, const R* xp, const R* yp, const R* Eprev, double ck1, double* lx,
               double* ly, double* lvx, double* lvy, double* lE, int64_t n);
+// forward step of tile-sorted particles (in place) with LDS windows for the field and
+// the new charge; items {tile x, tile y, split} over the tile offsets `toff`
+template <typename R>
+void adj_push_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff, R* x,
+                  R* y, R* vx, R* vy, const R* E, double c, unsigned long long* hi,
+                  unsigned long long* lo);
+// reverse step of tile-sorted particles; the E sensitivity is summed in an LDS window
+template <typename R>
+void adj_pass_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff,
+                  const R* xk, const R* yk, const double* lrho, double cq, const R* xp,
+                  const R* yp, const R* Eprev, double ck1, double* lx, double* ly,
+                  double* lvx, double* lvy, double* lE);
+// four particle arrays between a checkpoint's order and the original one:
+// to_orig: t[k][id[j]] = a[k][j];  from_orig: a[k][i] = t[k][id[i]]  (t: 4 x n)
+void adj_to_orig(hipStream_t s, const int* id, double* a0, double* a1, double* a2, double* a3,
+                 double* t, int64_t n);
+void adj_from_orig(hipStream_t s, const int* id, const double* t, double* a0, double* a1,
+                   double* a2, double* a3, int64_t n);
+void adj_iota(hipStream_t s, int* id, int64_t n);
 template <typename R>
 void adj_seed(hipStream_t s, const AdjGrid& g, const R* E, double sc, double* lE);
 template <typename R>

@@ -169,6 +169,7 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_zero(double* __restrict__ a, lon
 
 // continuous CIC weights and their gradients at x (double arithmetic)
 struct CicD {
+  int cx, cy;                        // cell
   int c00, c10, c01, c11;            // node indices (periodic)
   double w[4], wx[4], wy[4];         // W, dW/dx, dW/dy in node order 00,10,01,11
 };
@@ -184,6 +185,7 @@ __device__ __forceinline__ CicD cic_d(R xr, R yr, const AdjGrid& g) {
   fy = fy < 0 ? 0 : (fy > 1 ? 1 : fy);
   const int cxp = (cx + 1 == g.nx) ? 0 : cx + 1, cyp = (cy + 1 == g.ny) ? 0 : cy + 1;
   CicD r;
+  r.cx = cx; r.cy = cy;
   r.c00 = cx * g.ny + cy; r.c10 = cxp * g.ny + cy; r.c01 = cx * g.ny + cyp; r.c11 = cxp * g.ny + cyp;
   const double ix = 1.0 / g.dx, iy = 1.0 / g.dy;
   r.w[0] = (1 - fx) * (1 - fy); r.w[1] = fx * (1 - fy); r.w[2] = (1 - fx) * fy; r.w[3] = fx * fy;
@@ -249,6 +251,235 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_pass(AdjGrid g, const R* __restr
     lx[p] = ax;
     ly[p] = ay;
   }
+}
+
+// ---- windowed forms.  The particles are counting-sorted by tile at every checkpoint
+// (stable, with their original indices), so within a segment the particles of a push item
+// stay near its tile: the field is read from an LDS window of the tile + halo, and the
+// deposits (int64 charge forward, FP64 E sensitivity in reverse) are summed in an LDS
+// window and flushed with one global atomic per touched node.  Particles that drifted out
+// of the window since the checkpoint use the global arrays directly.  Per particle the
+// arithmetic is that of k_adj_push / k_adj_pass.
+
+// item {tile x, tile y, split} -> particle range [f0, f1) and window origin
+__device__ __forceinline__ void adj_item(const AdjGrid& g, const int4 it,
+                                         const int* __restrict__ toff, int& f0, int& f1,
+                                         int& x0, int& y0) {
+  const int tb = it.x * g.tiles_y + it.y;
+  const long long s0 = toff[tb], F = toff[tb + 1] - s0;
+  f0 = (int)(s0 + F * it.z / g.splits);
+  f1 = (int)(s0 + F * (it.z + 1) / g.splits);
+  x0 = it.x * g.tile - g.halo;
+  y0 = it.y * g.tile - g.halo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_adj_push_win(AdjGrid g,
+                                                         const int4* __restrict__ items,
+                                                         const int* __restrict__ toff,
+                                                         R* __restrict__ x, R* __restrict__ y,
+                                                         R* __restrict__ vx,
+                                                         R* __restrict__ vy,
+                                                         const R* __restrict__ Epad, double c,
+                                                         unsigned long long* hi,
+                                                         unsigned long long* lo) {
+  extern __shared__ unsigned long long wsh[];
+  const int W = g.win, nw = W * W;
+  constexpr int NQ = (sizeof(R) == 8) ? 2 : 1;  // int64 words per node (hi, lo)
+  unsigned long long* whi = wsh;
+  unsigned long long* wlo = wsh + nw;            // FP64 particles only
+  R2T<R>* ew = reinterpret_cast<R2T<R>*>(wsh + NQ * nw);
+  int f0, f1, x0, y0;
+  adj_item(g, items[blockIdx.x], toff, f0, f1, x0, y0);
+  const int ny1 = g.ny + 1;
+  const R2T<R>* E = reinterpret_cast<const R2T<R>*>(Epad);
+  for (int i = threadIdx.x; i < NQ * nw; i += PIC_WG) wsh[i] = 0ull;
+  for (int i = threadIdx.x; i < nw; i += PIC_WG) {
+    const int r = i / W, cc = i - r * W;
+    ew[i] = E[(long long)pwrap(x0 + r, g.nx) * ny1 + pwrap(y0 + cc, g.ny)];
+  }
+  __syncthreads();
+  const Geo<R> q = geo_of<R>(g);
+  const R Lx = (R)g.lx, Ly = (R)g.ly;
+  const R hk = (R)(c * g.dt * g.qm), dtr = (R)g.dt;
+  const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
+  for (int p = f0 + threadIdx.x; p < f1; p += PIC_WG) {
+    R xr = x[p], yr = y[p], vxr = vx[p], vyr = vy[p];
+    int cx, cy;
+    R fx, fy;
+    locate_w(xr, q.idx, g.nx, cx, fx);
+    locate_w(yr, q.idy, g.ny, cy, fy);
+    const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
+    const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
+    R2T<R> e00, e01, e10, e11;
+    int rx = pwrap(cx - x0, g.nx), ry = pwrap(cy - y0, g.ny);
+    if (rx < W - 1 && ry < W - 1) {
+      const int i0 = rx * W + ry;
+      e00 = ew[i0]; e01 = ew[i0 + 1]; e10 = ew[i0 + W]; e11 = ew[i0 + W + 1];
+    } else {
+      const long long i00 = (long long)cx * ny1 + cy;
+      e00 = E[i00]; e01 = E[i00 + 1]; e10 = E[i00 + ny1]; e11 = E[i00 + ny1 + 1];
+    }
+    R epx = (gx0 * gy0) * e00.x;
+    epx = epx + (gx1 * gy0) * e10.x;
+    epx = epx + (gx0 * gy1) * e01.x;
+    epx = epx + (gx1 * gy1) * e11.x;
+    R epy = (gx0 * gy0) * e00.y;
+    epy = epy + (gx1 * gy0) * e10.y;
+    epy = epy + (gx0 * gy1) * e01.y;
+    epy = epy + (gx1 * gy1) * e11.y;
+    vxr = vxr + hk * epx;
+    vyr = vyr + hk * epy;
+    xr = wrap_pos((R)(xr + dtr * vxr), Lx);
+    yr = wrap_pos((R)(yr + dtr * vyr), Ly);
+    x[p] = xr;
+    y[p] = yr;
+    vx[p] = vxr;
+    vy[p] = vyr;
+    long long w[4];
+    deposit_weights<R>(xr, yr, q, cx, cy, w);
+    rx = pwrap(cx - x0, g.nx);
+    ry = pwrap(cy - y0, g.ny);
+    if (rx < W - 1 && ry < W - 1) {
+      const int i0 = rx * W + ry;
+      const int off[4] = {0, W, 1, W + 1};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (sizeof(R) == 8) {
+          atomicAdd(&whi[i0 + off[k]], (unsigned long long)(w[k] >> 30));
+          atomicAdd(&wlo[i0 + off[k]], (unsigned long long)(w[k] & ((1ll << 30) - 1)));
+        } else {
+          atomicAdd(&whi[i0 + off[k]], (unsigned long long)w[k]);
+        }
+      }
+    } else {
+      deposit_global<R>(hi, lo, cx, cy, w, q);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nw; i += PIC_WG) {
+    const unsigned long long v = whi[i];
+    const unsigned long long vl = (sizeof(R) == 8) ? wlo[i] : 0ull;
+    if (v | vl) {
+      const int r = i / W, cc = i - r * W;
+      const long long nd = (long long)pwrap(x0 + r, g.nx) * g.ny + pwrap(y0 + cc, g.ny);
+      if (v) atomicAdd(&hi[nd], v);
+      if (sizeof(R) == 8 && vl) atomicAdd(&lo[nd], vl);
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(PIC_WG) void k_adj_pass_win(AdjGrid g,
+                                                         const int4* __restrict__ items,
+                                                         const int* __restrict__ toff,
+                                                         const R* __restrict__ xk,
+                                                         const R* __restrict__ yk,
+                                                         const double* __restrict__ lrho,
+                                                         double cq, const R* __restrict__ xp,
+                                                         const R* __restrict__ yp,
+                                                         const R* __restrict__ Eprev,
+                                                         double ck1, double* __restrict__ lx,
+                                                         double* __restrict__ ly,
+                                                         double* __restrict__ lvx,
+                                                         double* __restrict__ lvy,
+                                                         double* __restrict__ lE) {
+  extern __shared__ double wl[];  // E sensitivity window: x components, then y
+  const int W = g.win, nw = W * W;
+  const int G = g.nx * g.ny;
+  int f0, f1, x0, y0;
+  adj_item(g, items[blockIdx.x], toff, f0, f1, x0, y0);
+  for (int i = threadIdx.x; i < 2 * nw; i += PIC_WG) wl[i] = 0.0;
+  __syncthreads();
+  const R2T<R>* E = reinterpret_cast<const R2T<R>*>(Eprev);
+  const int ny1 = g.ny + 1;
+  for (int p = f0 + threadIdx.x; p < f1; p += PIC_WG) {
+    double ax = lx[p], ay = ly[p];
+    {
+      const CicD c = cic_d<R>(xk[p], yk[p], g);
+      const int nd[4] = {c.c00, c.c10, c.c01, c.c11};
+      double sx = 0.0, sy = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double l = lrho[nd[q]];
+        sx += c.wx[q] * l;
+        sy += c.wy[q] * l;
+      }
+      ax += cq * sx;
+      ay += cq * sy;
+    }
+    const double bx = lvx[p] + g.dt * ax, by = lvy[p] + g.dt * ay;
+    const CicD c = cic_d<R>(xp[p], yp[p], g);
+    const int nd[4] = {c.c00, c.c10, c.c01, c.c11};
+    const int rx = pwrap(c.cx - x0, g.nx), ry = pwrap(c.cy - y0, g.ny);
+    const bool inw = rx < W - 1 && ry < W - 1;
+    const int i0 = rx * W + ry;
+    const int off[4] = {0, W, 1, W + 1};
+    double exx = 0, exy = 0, eyx = 0, eyy = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = nd[q] / g.ny, j = nd[q] % g.ny;
+      const R2T<R> e = E[(long long)i * ny1 + j];
+      exx += c.wx[q] * (double)e.x; exy += c.wy[q] * (double)e.x;
+      eyx += c.wx[q] * (double)e.y; eyy += c.wy[q] * (double)e.y;
+      if (inw) {
+        atomicAdd(&wl[i0 + off[q]], ck1 * c.w[q] * bx);
+        atomicAdd(&wl[nw + i0 + off[q]], ck1 * c.w[q] * by);
+      } else {
+        atomicAdd(&lE[nd[q]], ck1 * c.w[q] * bx);
+        atomicAdd(&lE[G + nd[q]], ck1 * c.w[q] * by);
+      }
+    }
+    ax += ck1 * (bx * exx + by * eyx);
+    ay += ck1 * (bx * exy + by * eyy);
+    lvx[p] = bx;
+    lvy[p] = by;
+    lx[p] = ax;
+    ly[p] = ay;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nw; i += PIC_WG) {
+    const double sx = wl[i], sy = wl[nw + i];
+    if (sx != 0.0 || sy != 0.0) {
+      const int r = i / W, cc = i - r * W;
+      const int nd = pwrap(x0 + r, g.nx) * g.ny + pwrap(y0 + cc, g.ny);
+      if (sx != 0.0) atomicAdd(&lE[nd], sx);
+      if (sy != 0.0) atomicAdd(&lE[G + nd], sy);
+    }
+  }
+}
+
+// particle arrays between a checkpoint's order and the original order
+__global__ __launch_bounds__(PIC_WG) void k_adj_to_orig(const int* __restrict__ id,
+                                                        const double* __restrict__ a0,
+                                                        const double* __restrict__ a1,
+                                                        const double* __restrict__ a2,
+                                                        const double* __restrict__ a3,
+                                                        double* __restrict__ t, int64_t n) {
+  for (int64_t j = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * PIC_WG) {
+    const int64_t o = id[j];
+    t[o] = a0[j]; t[n + o] = a1[j]; t[2 * n + o] = a2[j]; t[3 * n + o] = a3[j];
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_adj_from_orig(const int* __restrict__ id,
+                                                          const double* __restrict__ t,
+                                                          double* __restrict__ a0,
+                                                          double* __restrict__ a1,
+                                                          double* __restrict__ a2,
+                                                          double* __restrict__ a3, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * PIC_WG) {
+    const int64_t o = id[i];
+    a0[i] = t[o]; a1[i] = t[n + o]; a2[i] = t[2 * n + o]; a3[i] = t[3 * n + o];
+  }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_adj_iota(int* __restrict__ id, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * PIC_WG)
+    id[i] = (int)i;
 }
 
 // lE = s * E (unpadded FP64 two-component array from the padded field)
@@ -437,6 +668,33 @@ void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const d
                                             lvy, lE, n);
 }
 template <typename R>
+void adj_push_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff, R* x,
+                  R* y, R* vx, R* vy, const R* E, double c, unsigned long long* hi,
+                  unsigned long long* lo) {
+  const size_t nw = (size_t)g.win * g.win;
+  const size_t lds = nw * ((sizeof(R) == 8 ? 2 : 1) * sizeof(unsigned long long) +
+                           2 * sizeof(R));
+  k_adj_push_win<R><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, x, y, vx, vy, E, c, hi, lo);
+}
+template <typename R>
+void adj_pass_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff,
+                  const R* xk, const R* yk, const double* lrho, double cq, const R* xp,
+                  const R* yp, const R* Eprev, double ck1, double* lx, double* ly,
+                  double* lvx, double* lvy, double* lE) {
+  const size_t lds = 2 * sizeof(double) * (size_t)g.win * g.win;
+  k_adj_pass_win<R><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, xk, yk, lrho, cq, xp, yp,
+                                                  Eprev, ck1, lx, ly, lvx, lvy, lE);
+}
+void adj_to_orig(hipStream_t s, const int* id, double* a0, double* a1, double* a2, double* a3,
+                 double* t, int64_t n) {
+  k_adj_to_orig<<<gridn(n), PIC_WG, 0, s>>>(id, a0, a1, a2, a3, t, n);
+}
+void adj_from_orig(hipStream_t s, const int* id, const double* t, double* a0, double* a1,
+                   double* a2, double* a3, int64_t n) {
+  k_adj_from_orig<<<gridn(n), PIC_WG, 0, s>>>(id, t, a0, a1, a2, a3, n);
+}
+void adj_iota(hipStream_t s, int* id, int64_t n) { k_adj_iota<<<gridn(n), PIC_WG, 0, s>>>(id, n); }
+template <typename R>
 void adj_seed(hipStream_t s, const AdjGrid& g, const R* E, double sc, double* lE) {
   k_adj_seed<R><<<gridn((long long)g.nx * g.ny), PIC_WG, 0, s>>>(g, E, sc, lE);
 }
@@ -474,6 +732,13 @@ void adj_to_r(hipStream_t s, const double* a, double sc, R* out, int64_t n) {
   template void adj_pass<R>(hipStream_t, const AdjGrid&, const R*, const R*, const double*,   \
                             double, const R*, const R*, const R*, double, double*, double*,  \
                             double*, double*, double*, int64_t);                             \
+  template void adj_push_win<R>(hipStream_t, const AdjGrid&, const int4*, const int*, R*, R*,  \
+                                R*, R*, const R*, double, unsigned long long*,               \
+                                unsigned long long*);                                        \
+  template void adj_pass_win<R>(hipStream_t, const AdjGrid&, const int4*, const int*,         \
+                                const R*, const R*, const double*, double, const R*,         \
+                                const R*, const R*, double, double*, double*, double*,       \
+                                double*, double*);                                           \
   template void adj_seed<R>(hipStream_t, const AdjGrid&, const R*, double, double*);          \
   template void adj_energy<R>(hipStream_t, const AdjGrid&, const R*, double*, int);           \
   template void adj_to_r<R>(hipStream_t, const double*, double, R*, int64_t);
