@@ -114,9 +114,9 @@ void record_forward(pic_adjoint& a) {
   const size_t n = (size_t)a.n;
   for (int k = 0; k < a.nsteps; ++k) {
     R* Ek = (R*)a.at(a.trE, (size_t)k * a.epad());
-    field(a, Ek);
     const int seg = k / a.K;
     const int* toff = a.toffs + (size_t)seg * (a.nb + 1);
+    if (k > 0) field(a, Ek);  // the charge of x_k (deposited by the previous push)
     if (k % a.K == 0) {
       // checkpoint = the state sorted by tile, with the original index of every slot
       R* ck = (R*)a.at(a.ckpt, (size_t)seg * 4 * n);
@@ -128,6 +128,10 @@ void record_forward(pic_adjoint& a) {
       R* cur[4] = {x, y, vx, vy};
       for (int q = 0; q < 4; ++q)
         (void)hipMemcpyAsync(cur[q], ck + q * n, a.rs * n, hipMemcpyDeviceToDevice, a.s);
+    }
+    if (k == 0) {  // the initial charge, deposited once the particles are sorted
+      adj_push_win<R>(a.s, a.g, a.items, toff, x, y, vx, vy, nullptr, 0.0, a.hi, a.lo);
+      field(a, Ek);
     }
     adj_push_win<R>(a.s, a.g, a.items, toff, x, y, vx, vy, Ek, kick(k), a.hi, a.lo);
   }
@@ -200,8 +204,8 @@ int forward_t(pic_adjoint& a, const double* amps, const void* vx0, const void* v
     int rc = capture(a, &a.gfwd, [&] { record_forward<R>(a); });
     if (rc) return rc;
   }
-  adj_init<R>(a.s, a.g, a.xl, a.yl, a.n, amps, a.mkx, a.mky, a.nm, (R*)a.x, (R*)a.y, a.hi,
-              a.lo);
+  adj_init<R>(a.s, a.g, a.xl, a.yl, a.n, amps, a.mkx, a.mky, a.nm, (R*)a.x, (R*)a.y, nullptr,
+              nullptr);
   PIC_HIP(hipMemcpyAsync(a.vx, vx0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
   PIC_HIP(hipMemcpyAsync(a.vy, vy0, a.rs * a.n, hipMemcpyDeviceToDevice, a.s));
   PIC_HIP(hipGraphLaunch(a.gfwd, a.s));
@@ -349,13 +353,15 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
     g.nitems = A.nb * g.splits;
     d.tile = T; d.halo = g.halo; d.tiles_x = g.nx / T; d.tiles_y = g.tiles_y;
     d.win = g.win; d.splits = g.splits; d.nbuckets = A.nb;
-    d.nchunks = (int)((n + PIC_CHUNK - 1) / PIC_CHUNK);
+    // sort chunks of 1024 particles: the adjoint's runs are small (config 5: 512 chunks)
+    d.schunk = 1024;
+    d.nchunks = (int)((n + d.schunk - 1) / d.schunk);
     d.pcap = (int)n; d.tbase = 0; d.toffbase = 0; d.pbase = 0;
     std::vector<int2> ts, pi;
     for (long long s = 0; s < (long long)A.nb * d.nchunks; s += PIC_CELL_CHUNK)
       ts.push_back(make_int2(0, (int)s));
     d.tscan_item_begin = 0; d.tscan_item_end = (int)ts.size();
-    for (int64_t s = 0; s < n; s += PIC_CHUNK) pi.push_back(make_int2(0, (int)s));
+    for (int64_t s = 0; s < n; s += d.schunk) pi.push_back(make_int2(0, (int)s));
     std::vector<int4> it;
     for (int tx = 0; tx < g.nx / T; ++tx)
       for (int ty = 0; ty < g.tiles_y; ++ty)

@@ -29,7 +29,8 @@ void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const d
               double cq, const R* xp, const R* yp, const R* Eprev, double ck1, double* lx,
               double* ly, double* lvx, double* lvy, double* lE, int64_t n);
 // forward step of tile-sorted particles (in place) with LDS windows for the field and
-// the new charge; items {tile x, tile y, split} over the tile offsets `toff`
+// the new charge; items {tile x, tile y, split} over the tile offsets `toff`.  E == NULL:
+// deposit the charge of the positions as they are, no push
 template <typename R>
 void adj_push_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff, R* x,
                   R* y, R* vx, R* vy, const R* E, double c, unsigned long long* hi,

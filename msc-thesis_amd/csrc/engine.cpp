@@ -143,6 +143,7 @@ int Ctx::build_items() {
     const long long preg = round_up(std::max<long long>(d.pcap, 1), PIC_CHUNK);
     d.nbuckets = d.tiles_x * d.tiles_y;
     d.nchunks = (int)(preg / PIC_CHUNK);
+    d.schunk = PIC_CHUNK;
     if (d.nbuckets > 4096) return fail(PIC_ERR_INVALID_CONFIG, "more than 4096 tiles: use a larger tile");
     if ((long long)d.nbuckets * d.nchunks >= (1ll << 31))
       return fail(PIC_ERR_INVALID_CONFIG, "tile table too large: use a larger tile");

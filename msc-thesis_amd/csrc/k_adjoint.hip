@@ -30,6 +30,7 @@ __device__ __forceinline__ Geo<R> geo_of(const AdjGrid& g) {
 }
 
 // x0 = xl + sum_m a_m cos(kx_m xl + ky_m yl) (wrapped), y0 = yl; deposits x0's charge
+// unless hi is NULL
 template <typename R>
 __global__ __launch_bounds__(PIC_WG) void k_adj_init(AdjGrid g, const double* __restrict__ xl,
                                                      const double* __restrict__ yl, int64_t n,
@@ -49,10 +50,12 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_init(AdjGrid g, const double* __
     const R yr = (R)b;
     x[p] = xr;
     y[p] = yr;
-    int cx, cy;
-    long long w[4];
-    deposit_weights<R>(xr, yr, q, cx, cy, w);
-    deposit_global<R>(hi, lo, cx, cy, w, q);
+    if (hi) {
+      int cx, cy;
+      long long w[4];
+      deposit_weights<R>(xr, yr, q, cx, cy, w);
+      deposit_global<R>(hi, lo, cx, cy, w, q);
+    }
   }
 }
 
@@ -273,7 +276,8 @@ __device__ __forceinline__ void adj_item(const AdjGrid& g, const int4 it,
   y0 = it.y * g.tile - g.halo;
 }
 
-template <typename R>
+// PUSH = false: deposit the charge of the positions as they are (no field, no motion)
+template <typename R, bool PUSH>
 __global__ __launch_bounds__(PIC_WG) void k_adj_push_win(AdjGrid g,
                                                          const int4* __restrict__ items,
                                                          const int* __restrict__ toff,
@@ -294,48 +298,53 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_push_win(AdjGrid g,
   const int ny1 = g.ny + 1;
   const R2T<R>* E = reinterpret_cast<const R2T<R>*>(Epad);
   for (int i = threadIdx.x; i < NQ * nw; i += PIC_WG) wsh[i] = 0ull;
-  for (int i = threadIdx.x; i < nw; i += PIC_WG) {
-    const int r = i / W, cc = i - r * W;
-    ew[i] = E[(long long)pwrap(x0 + r, g.nx) * ny1 + pwrap(y0 + cc, g.ny)];
-  }
+  if (PUSH)
+    for (int i = threadIdx.x; i < nw; i += PIC_WG) {
+      const int r = i / W, cc = i - r * W;
+      ew[i] = E[(long long)pwrap(x0 + r, g.nx) * ny1 + pwrap(y0 + cc, g.ny)];
+    }
   __syncthreads();
   const Geo<R> q = geo_of<R>(g);
   const R Lx = (R)g.lx, Ly = (R)g.ly;
   const R hk = (R)(c * g.dt * g.qm), dtr = (R)g.dt;
   const R qs = (R)(1.0 / (double)(1 << Fix<R>::QB));
   for (int p = f0 + threadIdx.x; p < f1; p += PIC_WG) {
-    R xr = x[p], yr = y[p], vxr = vx[p], vyr = vy[p];
-    int cx, cy;
-    R fx, fy;
-    locate_w(xr, q.idx, g.nx, cx, fx);
-    locate_w(yr, q.idy, g.ny, cy, fy);
-    const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
-    const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
-    R2T<R> e00, e01, e10, e11;
-    int rx = pwrap(cx - x0, g.nx), ry = pwrap(cy - y0, g.ny);
-    if (rx < W - 1 && ry < W - 1) {
-      const int i0 = rx * W + ry;
-      e00 = ew[i0]; e01 = ew[i0 + 1]; e10 = ew[i0 + W]; e11 = ew[i0 + W + 1];
-    } else {
-      const long long i00 = (long long)cx * ny1 + cy;
-      e00 = E[i00]; e01 = E[i00 + 1]; e10 = E[i00 + ny1]; e11 = E[i00 + ny1 + 1];
+    R xr = x[p], yr = y[p];
+    int cx, cy, rx, ry;
+    if (PUSH) {
+      R vxr = vx[p], vyr = vy[p];
+      R fx, fy;
+      locate_w(xr, q.idx, g.nx, cx, fx);
+      locate_w(yr, q.idy, g.ny, cy, fy);
+      const R gx1 = (R)qweight(fx) * qs, gy1 = (R)qweight(fy) * qs;
+      const R gx0 = (R)1 - gx1, gy0 = (R)1 - gy1;
+      R2T<R> e00, e01, e10, e11;
+      rx = pwrap(cx - x0, g.nx);
+      ry = pwrap(cy - y0, g.ny);
+      if (rx < W - 1 && ry < W - 1) {
+        const int i0 = rx * W + ry;
+        e00 = ew[i0]; e01 = ew[i0 + 1]; e10 = ew[i0 + W]; e11 = ew[i0 + W + 1];
+      } else {
+        const long long i00 = (long long)cx * ny1 + cy;
+        e00 = E[i00]; e01 = E[i00 + 1]; e10 = E[i00 + ny1]; e11 = E[i00 + ny1 + 1];
+      }
+      R epx = (gx0 * gy0) * e00.x;
+      epx = epx + (gx1 * gy0) * e10.x;
+      epx = epx + (gx0 * gy1) * e01.x;
+      epx = epx + (gx1 * gy1) * e11.x;
+      R epy = (gx0 * gy0) * e00.y;
+      epy = epy + (gx1 * gy0) * e10.y;
+      epy = epy + (gx0 * gy1) * e01.y;
+      epy = epy + (gx1 * gy1) * e11.y;
+      vxr = vxr + hk * epx;
+      vyr = vyr + hk * epy;
+      xr = wrap_pos((R)(xr + dtr * vxr), Lx);
+      yr = wrap_pos((R)(yr + dtr * vyr), Ly);
+      x[p] = xr;
+      y[p] = yr;
+      vx[p] = vxr;
+      vy[p] = vyr;
     }
-    R epx = (gx0 * gy0) * e00.x;
-    epx = epx + (gx1 * gy0) * e10.x;
-    epx = epx + (gx0 * gy1) * e01.x;
-    epx = epx + (gx1 * gy1) * e11.x;
-    R epy = (gx0 * gy0) * e00.y;
-    epy = epy + (gx1 * gy0) * e10.y;
-    epy = epy + (gx0 * gy1) * e01.y;
-    epy = epy + (gx1 * gy1) * e11.y;
-    vxr = vxr + hk * epx;
-    vyr = vyr + hk * epy;
-    xr = wrap_pos((R)(xr + dtr * vxr), Lx);
-    yr = wrap_pos((R)(yr + dtr * vyr), Ly);
-    x[p] = xr;
-    y[p] = yr;
-    vx[p] = vxr;
-    vy[p] = vyr;
     long long w[4];
     deposit_weights<R>(xr, yr, q, cx, cy, w);
     rx = pwrap(cx - x0, g.nx);
@@ -674,7 +683,12 @@ void adj_push_win(hipStream_t s, const AdjGrid& g, const int4* items, const int*
   const size_t nw = (size_t)g.win * g.win;
   const size_t lds = nw * ((sizeof(R) == 8 ? 2 : 1) * sizeof(unsigned long long) +
                            2 * sizeof(R));
-  k_adj_push_win<R><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, x, y, vx, vy, E, c, hi, lo);
+  if (E)
+    k_adj_push_win<R, true><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, x, y, vx, vy, E, c,
+                                                          hi, lo);
+  else
+    k_adj_push_win<R, false><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, x, y, vx, vy, E, c,
+                                                           hi, lo);
 }
 template <typename R>
 void adj_pass_win(hipStream_t s, const AdjGrid& g, const int4* items, const int* toff,

@@ -351,7 +351,7 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict
 
 // ------------------------------------------------------------ tile sort of the step
 // Stable counting sort of the particles by tile (sort bucket):
-//   1. k_tile_hist    per PIC_CHUNK chunk: count of each bucket -> table[b][chunk]
+//   1. k_tile_hist    per chunk of schunk particles: count of each bucket -> table[b][chunk]
 //   2. k_tscan_*      exclusive scan of the table in bucket-major order (per run)
 //   3. k_tile_scatter one wavefront per chunk walks its particles in index order; lanes
 //                     with equal buckets get consecutive slots (ballot ranks), so the
@@ -384,11 +384,11 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
   const Geo<R> g = make_geo<R>(rd.nx, rd.ny, rd.lx, rd.ly);
   const int np = st[it.x].np;
   const int nb = rd.nbuckets;
-  const int chunk = it.y / PIC_CHUNK;
+  const int chunk = it.y / rd.schunk;
   for (int b = threadIdx.x; b < nb; b += PIC_WG) h[b] = 0;
   __syncthreads();
-  const int p1 = min(it.y + PIC_CHUNK, np);
-  // four particles per thread per pass, loads first (PIC_CHUNK = 16 * PIC_WG)
+  const int p1 = min(it.y + rd.schunk, np);
+  // four particles per thread per pass, loads first
   for (int q = it.y + threadIdx.x; q < p1; q += 4 * PIC_WG) {
     R xs[4], ys[4];
 #pragma unroll
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
   const Geo<R> g = make_geo<R>(rd.nx, rd.ny, rd.lx, rd.ly);
   const int np = st[it.x].np;
   const int nb = rd.nbuckets;
-  const int chunk = it.y / PIC_CHUNK;
+  const int chunk = it.y / rd.schunk;
   const int lane = threadIdx.x;
   for (int b = lane; b < nb; b += 64) {
     const int v = table[rd.tbase + (long long)b * rd.nchunks + chunk];
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
   }
   __syncthreads();
   const long long pb = rd.pbase;
-  const int p1 = min(it.y + PIC_CHUNK, np);
+  const int p1 = min(it.y + rd.schunk, np);
   // SCAT_U groups of 64 particles per pass: all their loads are issued first (the wave's
   // memory latency is paid once per pass), then the groups are ranked in index order
   constexpr int SCAT_U = 4;

@@ -40,6 +40,7 @@ int check_grid(const pic_grid* g) {
 RunDesc op_desc(int nx, int ny, double lx, double ly, long long pcap) {
   RunDesc d;
   std::memset(&d, 0, sizeof(d));
+  d.schunk = PIC_CHUNK;
   d.nx = nx; d.ny = ny;
   int l = 0;
   while ((1 << l) < nx) ++l;

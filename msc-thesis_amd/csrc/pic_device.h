@@ -38,7 +38,7 @@ struct RunDesc {
   // behind; the push of the sort step scatters them tile-major into the other buffer
   int fchunks;                       // push items x PUSH_NW (0: fused sort off)
   int fscan_item_begin, fscan_item_end;
-  int fpad;
+  int schunk;                        // particles per chunk of the separate tile sort
   long long ftbase;                  // base of the run's bucket-major count table
   double lx, ly, dx, dy;
   double dt, qm, qp;
