@@ -35,7 +35,7 @@ struct pic_adjoint {
   void *trx = nullptr, *try_ = nullptr;       // (K+1) x n
   void* trE = nullptr;                        // nsteps x padded field (E_0 .. E_{N-1})
   double *lx = nullptr, *ly = nullptr, *lvx = nullptr, *lvy = nullptr;
-  double *lE = nullptr, *lrho = nullptr, *rho = nullptr;
+  double *lE = nullptr, *lrho = nullptr;
   unsigned long long *hi = nullptr, *lo = nullptr;
   void* E = nullptr;
   double2 *S = nullptr, *Sx = nullptr, *Sy = nullptr, *A = nullptr, *tw = nullptr;
@@ -86,11 +86,9 @@ namespace {
 
 constexpr int kEnergyBlocks = 256;
 
-// charge of the fixed-point sums (which it clears) -> E at `E` (padded, particle precision)
+// charge of the fixed-point sums (the row pass reads and clears them) -> E at `E` (padded,
+// particle precision)
 void field(pic_adjoint& a, void* E) {
-  const long long G = (long long)a.g.nx * a.g.ny;
-  const double scale = a.qp / (a.g.dx * a.g.dy) * (a.fp64 ? 1.0 : std::ldexp(1.0, -36));
-  adj_rho(a.s, G, a.fp64, scale, a.hi, a.lo, a.rho);
   FieldArgs f;
   std::memset(&f, 0, sizeof(f));
   f.runs = a.dd; f.st = a.st;
@@ -100,7 +98,9 @@ void field(pic_adjoint& a, void* E) {
   f.row_items = a.rw; f.n_row_items = a.nrw;
   f.nruns = 1; f.fp64 = a.fp64; f.want_phi = 0; f.max_nx = a.g.nx; f.max_ny = a.g.ny;
   f.row_lds = a.row_lds;
-  f.rho_in = a.rho;
+  f.rho = a.hi; f.rho_w = a.hi; f.rho_lo = a.lo; f.rho_lo_w = a.lo;
+  f.gsum = (long long)a.g.nx * a.g.ny;
+  f.clear_read = 1;
   launch_field(a.s, f, nullptr);
 }
 
@@ -299,7 +299,6 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   PIC_HIP(A.get(&A.lvy, n));
   PIC_HIP(A.get(&A.lE, 2 * G));
   PIC_HIP(A.get(&A.lrho, G));
-  PIC_HIP(A.get(&A.rho, G));
   PIC_HIP(A.get(&A.hi, G));
   PIC_HIP(A.get(&A.lo, G));
   PIC_HIP(A.getb(&A.E, A.rs * A.epad()));
@@ -332,6 +331,7 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   std::memset(&d, 0, sizeof(d));
   d.nx = g.nx; d.ny = g.ny; d.lognx = g.lognx; d.logny = g.logny;
   d.lx = g.lx; d.ly = g.ly; d.dx = g.dx; d.dy = g.dy;
+  d.rho_scale = a->qp / (g.dx * g.dy) * (a->fp64 ? 1.0 : std::ldexp(1.0, -36));
   fft_groups(g.nx, g.nx, g.ny, &d.fft_pp, &d.fft_rf);
   A.row_lds = std::max(d.fft_pp, d.fft_rf) * g.ny;
   d.row_item_begin = 0; d.row_item_end = g.nx / d.fft_rf;

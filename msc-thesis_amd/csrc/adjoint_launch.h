@@ -21,8 +21,6 @@ void adj_push(hipStream_t s, const AdjGrid& g, const R* xi, const R* yi, R* xo, 
 template <typename R>
 void adj_restore(hipStream_t s, const AdjGrid& g, const R* ck, int64_t n, R* x, R* y, R* vx,
                  R* vy, unsigned long long* hi, unsigned long long* lo);
-void adj_rho(hipStream_t s, long long G, int fp64, double scale, unsigned long long* hi,
-             unsigned long long* lo, double* rho);
 void adj_zero(hipStream_t s, double* a, long long n);
 template <typename R>
 void adj_pass(hipStream_t s, const AdjGrid& g, const R* xk, const R* yk, const double* lrho,

@@ -144,26 +144,6 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_restore(AdjGrid g, const R* __re
   }
 }
 
-// charge density from the fixed-point sums (scale = qp/dA, times 2^-36 for FP32 words);
-// the sums are zeroed for the next deposit
-__global__ __launch_bounds__(PIC_WG) void k_adj_rho(long long G, int fp64, double scale,
-                                                    unsigned long long* hi,
-                                                    unsigned long long* lo,
-                                                    double* __restrict__ rho) {
-  for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < G;
-       q += (long long)gridDim.x * PIC_WG) {
-    const long long h = (long long)hi[q];
-    hi[q] = 0ull;
-    if (fp64) {
-      const long long l = (long long)lo[q];
-      lo[q] = 0ull;
-      rho[q] = ((double)h * 9.313225746154785e-10 + (double)l * 8.673617379884035e-19) * scale;
-    } else {
-      rho[q] = (double)h * scale;
-    }
-  }
-}
-
 __global__ __launch_bounds__(PIC_WG) void k_adj_zero(double* __restrict__ a, long long n) {
   for (long long q = blockIdx.x * (long long)PIC_WG + threadIdx.x; q < n;
        q += (long long)gridDim.x * PIC_WG)
@@ -661,10 +641,6 @@ template <typename R>
 void adj_restore(hipStream_t s, const AdjGrid& g, const R* ck, int64_t n, R* x, R* y, R* vx,
                  R* vy, unsigned long long* hi, unsigned long long* lo) {
   k_adj_restore<R><<<gridn(n), PIC_WG, 0, s>>>(g, ck, n, x, y, vx, vy, hi, lo);
-}
-void adj_rho(hipStream_t s, long long G, int fp64, double scale, unsigned long long* hi,
-             unsigned long long* lo, double* rho) {
-  k_adj_rho<<<gridn(G), PIC_WG, 0, s>>>(G, fp64, scale, hi, lo, rho);
 }
 void adj_zero(hipStream_t s, double* a, long long n) {
   k_adj_zero<<<gridn(n), PIC_WG, 0, s>>>(a, n);

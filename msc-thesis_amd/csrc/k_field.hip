@@ -50,9 +50,10 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   } else {
     const int par = (int)(a.st[it.x].step & 1);
     const unsigned long long* rh = a.rho + par * a.gsum;
-    unsigned long long* zw = a.rho_w + (1 - par) * a.gsum;
+    const int zpar = a.clear_read ? par : 1 - par;
+    unsigned long long* zw = a.rho_w + zpar * a.gsum;
     const unsigned long long* rl = a.fp64 ? a.rho_lo + par * a.gsum : nullptr;
-    unsigned long long* zl = a.fp64 ? a.rho_lo_w + (1 - par) * a.gsum : nullptr;
+    unsigned long long* zl = a.fp64 ? a.rho_lo_w + zpar * a.gsum : nullptr;
     for (int t = threadIdx.x; t < PP * ny; t += PIC_WG) {
       const int q = t >> logny, j = t & (ny - 1);
       const long long e0 = g0 + (long long)(2 * q) * ny + j, e1 = e0 + ny;

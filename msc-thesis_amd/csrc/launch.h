@@ -181,6 +181,8 @@ struct FieldArgs {
   int row_lds;                       // complex values per row-pass workgroup: the largest
                                      // max(fft_pp, fft_rf) * ny over the runs
   const double* rho_in;              // op mode: FP64 density input instead of rho (may be NULL)
+  int clear_read;                    // 1: zero the charge words that are read (one buffer);
+                                     // 0: zero the other parity's (the next push's target)
 };
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev /* 4 events or NULL */);
 void launch_op_efield(hipStream_t s, const double* phi, double* ex, double* ey, int nx, int ny,
