@@ -34,7 +34,6 @@ struct pic_adjoint {
   void* ckpt = nullptr;                       // nseg x 4 x n
   void *trx = nullptr, *try_ = nullptr;       // (K+1) x n
   void* trE = nullptr;                        // nsteps x padded field (E_0 .. E_{N-1})
-  double *lx = nullptr, *ly = nullptr, *lvx = nullptr, *lvy = nullptr;
   double *lE = nullptr, *lrho = nullptr;
   unsigned long long *hi = nullptr, *lo = nullptr;
   void* E = nullptr;
@@ -45,14 +44,17 @@ struct pic_adjoint {
   int nrp = 0, ncl = 0, nrw = 0;
   // tile sort at the checkpoints and windowed kernels
   int nb = 0;                                 // tiles
-  int* ids = nullptr;                         // nseg x n: original index of each slot
-  int* iota = nullptr;                        // n: the original order
+  int* ids = nullptr;                         // nseg x n: per slot of segment s, its slot in
+                                              // the order before the sort (segment s - 1, or
+                                              // the original order for s = 0)
+  int* iota = nullptr;                        // n: 0, 1, ..., n - 1
   int* toffs = nullptr;                       // nseg x (nb + 1) tile offsets
   int4* items = nullptr;                      // {tile x, tile y, split} push / pass items
   int2* pitems = nullptr;                     // sort chunk items
   int npitems = 0;
   SortBufs sb;
-  double* tmp = nullptr;                      // 4 x n: adjoint arrays in original order
+  double* lam[2] = {nullptr, nullptr};        // 4 x n each: (lx, ly, lvx, lvy) of a segment
+  double* res = nullptr;                      // the one that ends in the original order
   int row_lds = 0;  // complex values per row-pass workgroup
   double* part = nullptr;
   bool forward_done = false;
@@ -118,9 +120,9 @@ void record_forward(pic_adjoint& a) {
     const int* toff = a.toffs + (size_t)seg * (a.nb + 1);
     if (k > 0) field(a, Ek);  // the charge of x_k (deposited by the previous push)
     if (k % a.K == 0) {
-      // checkpoint = the state sorted by tile, with the original index of every slot
+      // checkpoint = the state sorted by tile; ids = each slot's slot before the sort
       R* ck = (R*)a.at(a.ckpt, (size_t)seg * 4 * n);
-      PartBufs<R> src{x, y, vx, vy, seg == 0 ? a.iota : a.ids + (size_t)(seg - 1) * n};
+      PartBufs<R> src{x, y, vx, vy, a.iota};
       PartBufs<R> dst{ck, ck + n, ck + 2 * n, ck + 3 * n, a.ids + (size_t)seg * n};
       SortBufs sb = a.sb;
       sb.toff = a.toffs + (size_t)seg * (a.nb + 1);
@@ -147,7 +149,9 @@ void record_backward(pic_adjoint& a) {
   const size_t n = (size_t)a.n;
   adj_seed<R>(a.s, a.g, (const R*)a.E, 2.0 * a.g.dx * a.g.dy, a.lE);
   adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
-  for (double* p : {a.lx, a.ly, a.lvx, a.lvy}) adj_zero(a.s, p, a.n);
+  int cur = 0;  // adjoint arrays of the segment being reversed: lam[cur]
+  adj_zero(a.s, a.lam[cur], 4 * a.n);
+  auto L = [&](int k) { return a.lam[cur] + (size_t)k * n; };
   const double cq = a.qp / (a.g.dx * a.g.dy);
   auto X = [&](int slot) { return (R*)a.at(a.trx, (size_t)slot * n); };
   auto Y = [&](int slot) { return (R*)a.at(a.try_, (size_t)slot * n); };
@@ -156,9 +160,10 @@ void record_backward(pic_adjoint& a) {
     const int k0 = seg * a.K, k1 = std::min(k0 + a.K, a.nsteps);
     const int* toff = a.toffs + (size_t)seg * (a.nb + 1);
     if (seg < a.nseg - 1) {
-      // adjoint arrays from the order of segment seg + 1 to the order of segment seg
-      adj_to_orig(a.s, a.ids + (size_t)(seg + 1) * n, a.lx, a.ly, a.lvx, a.lvy, a.tmp, a.n);
-      adj_from_orig(a.s, a.ids + (size_t)seg * n, a.tmp, a.lx, a.ly, a.lvx, a.lvy, a.n);
+      // adjoint arrays from the order of segment seg + 1 to the order of segment seg: slot d
+      // of seg + 1 came from slot ids[seg + 1][d] of seg (mostly nearby: the same tile)
+      adj_reorder(a.s, a.ids + (size_t)(seg + 1) * n, a.lam[cur], a.lam[1 - cur], a.n);
+      cur = 1 - cur;
     }
     adj_restore<R>(a.s, a.g, (const R*)a.at(a.ckpt, (size_t)seg * 4 * n), a.n, X(0), Y(0),
                    (R*)a.vx, (R*)a.vy, nullptr, nullptr);
@@ -170,16 +175,16 @@ void record_backward(pic_adjoint& a) {
     for (int k = k1; k > k0; --k) {
       const int sl = k - k0;
       adj_pass_win<R>(a.s, a.g, a.items, toff, X(sl), Y(sl), a.lrho, cq, X(sl - 1), Y(sl - 1),
-                      Ek(k - 1), kick(k - 1) * a.g.dt * a.g.qm, a.lx, a.ly, a.lvx, a.lvy,
-                      a.lE);
+                      Ek(k - 1), kick(k - 1) * a.g.dt * a.g.qm, L(0), L(1), L(2), L(3), a.lE);
       adj_mt(a.s, a.g, a.lE, a.A, a.lrho, a.tw);
     }
   }
   // deposit Jacobian of step 0 at x_0 (slot 0 of segment 0)
-  adj_pass<R>(a.s, a.g, X(0), Y(0), a.lrho, cq, nullptr, nullptr, nullptr, 0.0, a.lx, a.ly,
-              a.lvx, a.lvy, a.lE, a.n);
-  // results in the original particle order: tmp = (lx, ly, lvx, lvy)
-  adj_to_orig(a.s, a.ids, a.lx, a.ly, a.lvx, a.lvy, a.tmp, a.n);
+  adj_pass<R>(a.s, a.g, X(0), Y(0), a.lrho, cq, nullptr, nullptr, nullptr, 0.0, L(0), L(1),
+              L(2), L(3), a.lE, a.n);
+  // results in the original particle order (segment 0 was sorted from it)
+  adj_reorder(a.s, a.ids, a.lam[cur], a.lam[1 - cur], a.n);
+  a.res = a.lam[1 - cur];
 }
 
 template <typename F>
@@ -227,9 +232,9 @@ int backward_t(pic_adjoint& a, double dloss, double* grad_amps, void* gvx, void*
     if (rc) return rc;
   }
   PIC_HIP(hipGraphLaunch(a.gbwd, a.s));
-  const double* lx0 = a.tmp;                 // original order
-  const double* lvx0 = a.tmp + 2 * (size_t)a.n;
-  const double* lvy0 = a.tmp + 3 * (size_t)a.n;
+  const double* lx0 = a.res;                 // original order
+  const double* lvx0 = a.res + 2 * (size_t)a.n;
+  const double* lvy0 = a.res + 3 * (size_t)a.n;
   if (grad_amps)
     adj_amp_grad(a.s, lx0, a.xl, a.yl, a.n, a.mkx, a.mky, a.nm, a.part, kEnergyBlocks, dloss,
                  grad_amps);
@@ -293,10 +298,8 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
   PIC_HIP(A.getb(&A.trx, A.rs * n * (A.K + 1)));
   PIC_HIP(A.getb(&A.try_, A.rs * n * (A.K + 1)));
   PIC_HIP(A.getb(&A.trE, A.rs * A.epad() * (size_t)nsteps));
-  PIC_HIP(A.get(&A.lx, n));
-  PIC_HIP(A.get(&A.ly, n));
-  PIC_HIP(A.get(&A.lvx, n));
-  PIC_HIP(A.get(&A.lvy, n));
+  PIC_HIP(A.get(&A.lam[0], 4 * (size_t)n));
+  PIC_HIP(A.get(&A.lam[1], 4 * (size_t)n));
   PIC_HIP(A.get(&A.lE, 2 * G));
   PIC_HIP(A.get(&A.lrho, G));
   PIC_HIP(A.get(&A.hi, G));
@@ -383,7 +386,6 @@ int pic_adjoint_create(const pic_config* cfg, const double* xl, const double* yl
     PIC_HIP(A.get(&A.ids, (size_t)A.nseg * n));
     PIC_HIP(A.get(&A.iota, n));
     PIC_HIP(A.get(&A.toffs, (size_t)A.nseg * (A.nb + 1)));
-    PIC_HIP(A.get(&A.tmp, 4 * (size_t)n));
     adj_iota(A.s, A.iota, n);
     PIC_HIP(hipStreamSynchronize(A.s));
   }

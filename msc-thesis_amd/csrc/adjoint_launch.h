@@ -39,12 +39,8 @@ void adj_pass_win(hipStream_t s, const AdjGrid& g, const int4* items, const int*
                   const R* xk, const R* yk, const double* lrho, double cq, const R* xp,
                   const R* yp, const R* Eprev, double ck1, double* lx, double* ly,
                   double* lvx, double* lvy, double* lE);
-// four particle arrays between a checkpoint's order and the original one:
-// to_orig: t[k][id[j]] = a[k][j];  from_orig: a[k][i] = t[k][id[i]]  (t: 4 x n)
-void adj_to_orig(hipStream_t s, const int* id, double* a0, double* a1, double* a2, double* a3,
-                 double* t, int64_t n);
-void adj_from_orig(hipStream_t s, const int* id, const double* t, double* a0, double* a1,
-                   double* a2, double* a3, int64_t n);
+// four particle arrays (a: 4 x n) into the order before a sort: b[k][from[d]] = a[k][d]
+void adj_reorder(hipStream_t s, const int* from, const double* a, double* b, int64_t n);
 void adj_iota(hipStream_t s, int* id, int64_t n);
 template <typename R>
 void adj_seed(hipStream_t s, const AdjGrid& g, const R* E, double sc, double* lE);

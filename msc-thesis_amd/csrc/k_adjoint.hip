@@ -438,30 +438,15 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_pass_win(AdjGrid g,
   }
 }
 
-// particle arrays between a checkpoint's order and the original order
-__global__ __launch_bounds__(PIC_WG) void k_adj_to_orig(const int* __restrict__ id,
-                                                        const double* __restrict__ a0,
-                                                        const double* __restrict__ a1,
-                                                        const double* __restrict__ a2,
-                                                        const double* __restrict__ a3,
-                                                        double* __restrict__ t, int64_t n) {
-  for (int64_t j = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; j < n;
-       j += (int64_t)gridDim.x * PIC_WG) {
-    const int64_t o = id[j];
-    t[o] = a0[j]; t[n + o] = a1[j]; t[2 * n + o] = a2[j]; t[3 * n + o] = a3[j];
-  }
-}
-
-__global__ __launch_bounds__(PIC_WG) void k_adj_from_orig(const int* __restrict__ id,
-                                                          const double* __restrict__ t,
-                                                          double* __restrict__ a0,
-                                                          double* __restrict__ a1,
-                                                          double* __restrict__ a2,
-                                                          double* __restrict__ a3, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * PIC_WG) {
-    const int64_t o = id[i];
-    a0[i] = t[o]; a1[i] = t[n + o]; a2[i] = t[2 * n + o]; a3[i] = t[3 * n + o];
+// four particle arrays into the order before a sort: b[k][from[d]] = a[k][d]
+__global__ __launch_bounds__(PIC_WG) void k_adj_reorder(const int* __restrict__ from,
+                                                        const double* __restrict__ a,
+                                                        double* __restrict__ b, int64_t n) {
+  for (int64_t d = blockIdx.x * (int64_t)PIC_WG + threadIdx.x; d < n;
+       d += (int64_t)gridDim.x * PIC_WG) {
+    const int64_t o = from[d];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k * n + o] = a[k * n + d];
   }
 }
 
@@ -675,13 +660,8 @@ void adj_pass_win(hipStream_t s, const AdjGrid& g, const int4* items, const int*
   k_adj_pass_win<R><<<g.nitems, PIC_WG, lds, s>>>(g, items, toff, xk, yk, lrho, cq, xp, yp,
                                                   Eprev, ck1, lx, ly, lvx, lvy, lE);
 }
-void adj_to_orig(hipStream_t s, const int* id, double* a0, double* a1, double* a2, double* a3,
-                 double* t, int64_t n) {
-  k_adj_to_orig<<<gridn(n), PIC_WG, 0, s>>>(id, a0, a1, a2, a3, t, n);
-}
-void adj_from_orig(hipStream_t s, const int* id, const double* t, double* a0, double* a1,
-                   double* a2, double* a3, int64_t n) {
-  k_adj_from_orig<<<gridn(n), PIC_WG, 0, s>>>(id, t, a0, a1, a2, a3, n);
+void adj_reorder(hipStream_t s, const int* from, const double* a, double* b, int64_t n) {
+  k_adj_reorder<<<gridn(n), PIC_WG, 0, s>>>(from, a, b, n);
 }
 void adj_iota(hipStream_t s, int* id, int64_t n) { k_adj_iota<<<gridn(n), PIC_WG, 0, s>>>(id, n); }
 template <typename R>
