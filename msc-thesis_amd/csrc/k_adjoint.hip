@@ -589,8 +589,17 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_amp_reduce(const double* __restr
                                                            int nblocks, int nm, double scale,
                                                            double* __restrict__ out) {
   if (threadIdx.x < nm) {
+    // in block order; the loads of eight blocks are issued before their adds
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(long long)b * nm + threadIdx.x];
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = part[(long long)(b + k) * nm + threadIdx.x];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; b < nblocks; ++b) s += part[(long long)b * nm + threadIdx.x];
     out[threadIdx.x] = scale * s;
   }
 }
