@@ -628,7 +628,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
       int d = 0;
       while (rem) {
         const int leader = __ffsll((long long)rem) - 1;
-        const int lk = __shfl(key[u], leader, 64);
+        const int lk = __builtin_amdgcn_readlane(key[u], leader);
         const unsigned long long m = __ballot(act && key[u] == lk);
         const int b0 = base[lk];
         if (act && key[u] == lk) d = b0 + __popcll(m & ((1ull << lane) - 1ull));
