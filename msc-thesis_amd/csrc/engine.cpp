@@ -104,7 +104,7 @@ Ctx::~Ctx() {
   F(toffs[0]); F(toffs[1]); F(sb.ftable); F(sb.fchunk_sums); F(d_fscan_items);
   F(d_pitems); F(d_citems); F(d_rowpair); F(d_cols); F(d_rows); F(d_push);
   F(d_segitems); F(d_tscan_items);
-  F(partials_push); F(partials_rows); F(tickets); F(diag);
+  F(partials_push); F(partials_rows); F(diag);
   F(d_coll_counts); F(birth_count); F(birth_parent); F(birth_buf); F(gmax); F(flag);
   for (auto& e : ev) if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -297,7 +297,6 @@ int Ctx::alloc() {
   PIC_HIP(dalloc(&sb.big_list, 2 * ctotal));
   PIC_HIP(dalloc(&sb.big_count, 1));
   PIC_HIP(dalloc(&sb.scratch, 2 * ptotal));
-  PIC_HIP(dalloc(&tickets, 2 * nruns));  // per-run tickets: push [0, nruns), field [nruns, 2 nruns)
   PIC_HIP(dalloc(&diag, (size_t)nruns * PIC_DIAG_CAP * 8));
   PIC_HIP(dalloc(&d_coll_counts, 4 * nruns));
   PIC_HIP(dalloc(&flag, 1));
@@ -320,7 +319,6 @@ int Ctx::alloc() {
   PIC_HIP(hipMemset(d_st, 0, sizeof(RunState) * nruns));
   PIC_HIP(hipMemset(rho, 0, sizeof(unsigned long long) * 2 * gsum));
   if (fp64) PIC_HIP(hipMemset(rho_lo, 0, sizeof(unsigned long long) * 2 * gsum));
-  PIC_HIP(hipMemset(tickets, 0, sizeof(unsigned int) * 2 * nruns));
   PIC_HIP(hipMemset(diag, 0, sizeof(double) * (size_t)nruns * PIC_DIAG_CAP * 8));
   PIC_HIP(hipMemset(d_coll_counts, 0, sizeof(long long) * 4 * nruns));
   PIC_HIP(hipMemset(E, 0, rs * 2 * esum));
@@ -342,7 +340,7 @@ FieldArgs Ctx::field_args(int want_phi) const {
   a.row_pair_items = d_rowpair; a.n_row_pair_items = (int)rowpair.size();
   a.col_items = d_cols; a.n_col_items = (int)cols.size();
   a.row_items = d_rows; a.n_row_items = (int)rows.size();
-  a.partials = partials_rows; a.ticket = tickets + nruns; a.diag = diag;
+  a.partials = partials_rows; a.diag = diag;
   a.nruns = nruns; a.fp64 = fp64; a.want_phi = want_phi;
   a.max_nx = max_nx; a.max_ny = max_ny;
   a.row_lds = 0;
@@ -363,7 +361,7 @@ PushArgs<R> Ctx::push_args(int mode, int sort_mode) const {
   a.off = toffs[cur];
   a.items = d_push; a.nitems = (int)push_items.size();
   a.nruns = nruns;
-  a.partials = partials_push; a.ticket = tickets; a.diag = diag;
+  a.partials = partials_push;
   a.mode = mode; a.lds_bytes = lds_push;
   a.sort_mode = fused ? sort_mode : 0;
   a.ftable = sb.ftable;
@@ -467,6 +465,7 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev) {
     else if (steps_since_sort == sort_every - 1) smode = 2;
   }
   launch_push_stage(0, smode);
+  launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
   if (smode == 1) {
     sb.toff = toffs[1 - cur];
     launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);

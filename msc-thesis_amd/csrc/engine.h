@@ -75,7 +75,6 @@ struct Ctx {
   int4* d_push = nullptr;
   double* partials_push = nullptr;
   double* partials_rows = nullptr;
-  unsigned int* tickets = nullptr;  // [2]
   double* diag = nullptr;
   int lds_push = 0;
   // collisions

@@ -134,7 +134,6 @@ template <typename R, int MODE>
 __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   extern __shared__ double2 lds[];
   __shared__ double red[4];
-  __shared__ unsigned int ticket_s;
   const int2 it = (MODE == 0) ? a.row_items[blockIdx.x] : a.row_pair_items[blockIdx.x];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
@@ -190,25 +189,10 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
       if (j == 0) E[(long long)nx * (ny + 1) + ny] = v;
     }
   }
-  if (!a.diag) return;
+  // field-energy partial of this row group (k_step_finish sums them per run in row order)
+  if (MODE != 0 || !a.partials) return;
   const double s = block_sum(en, red);
-  if (threadIdx.x == 0) {
-    a.partials[blockIdx.x] = s;
-    __threadfence();
-    ticket_s = atomicAdd(a.ticket + it.x, 1u);
-  }
-  __syncthreads();
-  // the last row group of this run sums the run's partials in row order
-  if (ticket_s != (unsigned int)(rd.row_item_end - rd.row_item_begin - 1)) return;
-  __threadfence();
-  double v = 0.0;
-  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) v += a.partials[q];
-  v = block_sum(v, red);
-  if (threadIdx.x == 0) {
-    const long long stp = a.st[it.x].step;
-    a.diag[(rd.dbase + (stp % PIC_DIAG_CAP)) * 8 + 0] = 0.5 * v * rd.dx * rd.dy;
-    a.ticket[it.x] = 0u;
-  }
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_op_efield(const double* __restrict__ phi,

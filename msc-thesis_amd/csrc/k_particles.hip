@@ -1014,6 +1014,49 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
   sort_tail(s, runs, st, nullptr, sb, citems, ncitems);
 }
 
+// End of a step, one workgroup per run: the diagnostics row of the step from the
+// per-workgroup partial sums of the field pass (field energy, in row-group order) and of
+// the push (kinetic energy and momentum, in item order), then the step counter.  Kernel
+// order makes the partials visible (no grid-wide fences in the producers).
+__global__ __launch_bounds__(PIC_WG) void k_step_finish(const RunDesc* __restrict__ runs,
+                                                        RunState* __restrict__ st,
+                                                        const double* __restrict__ ppush,
+                                                        const double* __restrict__ prows,
+                                                        double* __restrict__ diag) {
+  __shared__ double red[4];
+  const int run = blockIdx.x;
+  const RunDesc rd = runs[run];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int i = rd.push_item_begin + threadIdx.x; i < rd.push_item_end; i += PIC_WG) {
+    s0 += ppush[3 * (long long)i + 0];
+    s1 += ppush[3 * (long long)i + 1];
+    s2 += ppush[3 * (long long)i + 2];
+  }
+  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) s3 += prows[q];
+  s0 = block_sum(s0, red);
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  s3 = block_sum(s3, red);
+  if (threadIdx.x == 0) {
+    RunState* sr = st + run;
+    const long long stp = sr->step;
+    const double mp = rd.qp / rd.qm;
+    double* row = diag + (rd.dbase + (stp % PIC_DIAG_CAP)) * 8;
+    row[0] = 0.5 * s3 * rd.dx * rd.dy;
+    row[1] = 0.5 * mp * s0;
+    row[2] = mp * s1;
+    row[3] = mp * s2;
+    row[5] = (double)sr->np;
+    sr->step = stp + 1;
+    sr->vsync = 0;
+  }
+}
+
+void launch_step_finish(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                        const double* ppush, const double* prows, double* diag) {
+  k_step_finish<<<nruns, PIC_WG, 0, s>>>(runs, st, ppush, prows, diag);
+}
+
 template <typename R>
 void launch_push(hipStream_t s, const PushArgs<R>& a) {
   if (a.nitems <= 0) return;

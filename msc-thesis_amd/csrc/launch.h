@@ -78,9 +78,7 @@ struct PushArgs {
   const int4* items;
   int nitems;
   int nruns;
-  double* partials;             // [nitems*3]
-  unsigned int* ticket;         // per run: arrival count of its workgroups (reset by the last)
-  double* diag;
+  double* partials;             // [nitems*3] per-workgroup kinetic energy, px, py
   int mode;                     // 0: gather+push+deposit, 1: deposit only
   int lds_bytes;
   // sort fused into the push (mode 0, tile items): 0 off; 1 count the tiles of the new
@@ -93,6 +91,9 @@ struct PushArgs {
 };
 template <typename R>
 void launch_push(hipStream_t s, const PushArgs<R>& a);
+// per run: diagnostics row of the step from the field and push partials, then step + 1
+void launch_step_finish(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,
+                        const double* ppush, const double* prows, double* diag);
 
 template <typename R>
 void launch_half_kick_out(hipStream_t s, const RunDesc* runs, RunState* st, int run,
@@ -171,8 +172,7 @@ struct FieldArgs {
   const int2* row_pair_items; int n_row_pair_items;
   const int2* col_items; int n_col_items;
   const int2* row_items; int n_row_items;
-  double* partials;                  // [n_row_items]
-  unsigned int* ticket;         // per run: arrival count of its workgroups (reset by the last)
+  double* partials;                  // [n_row_items] per-row-group field energy (may be NULL)
   double* diag;
   int nruns;
   int fp64;                          // particle precision (E type, rho words)
