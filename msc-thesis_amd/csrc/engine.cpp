@@ -178,15 +178,15 @@ int Ctx::build_items() {
   if (lds_push > 160 * 1024) return fail(PIC_ERR_INVALID_CONFIG, "deposit window exceeds LDS");
   // Sort fused into the push: needs no births or cell lists (no collisions), a cadence of
   // at least two steps (the count is taken one step ahead), and room in LDS for one
-  // counter per tile and wave.
-  const int fused_lds = PUSH_NW * maxb * (int)sizeof(int);
+  // counter per tile (int) and two per tile and wave (16-bit pass counts).
+  const int fused_lds = (PUSH_NW + 1) * maxb * (int)sizeof(int);
   fused = !any_mcc && !any_dsmc && sort_every >= 2 && maxb <= 512 &&
           lds_push + fused_lds <= 160 * 1024;
   fscan_items.clear();
   long long fsum = 0;
   for (int r = 0; r < nruns && fused; ++r) {
     RunDesc& d = rd[r];
-    d.fchunks = (d.push_item_end - d.push_item_begin) * PUSH_NW;
+    d.fchunks = d.push_item_end - d.push_item_begin;
     d.ftbase = fsum;
     fsum += (long long)d.nbuckets * d.fchunks;
     d.fscan_item_begin = (int)fscan_items.size();

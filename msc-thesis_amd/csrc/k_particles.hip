@@ -464,9 +464,9 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict_
   }
 }
 
-// ---- scan of the fused sort's count table.  Row-major: row c (one wave of the push grid)
-// holds that wave's count for every tile, so each wave reads and writes one contiguous
-// row.  The destination slot of (tile b, row c) is
+// ---- scan of the fused sort's count table.  Row-major: row c (one item of the push grid)
+// holds that item's count for every tile, so each push workgroup reads and writes one
+// contiguous row.  The destination slot of (tile b, row c) is
 //   base[b] + sum_{c' < c} count[c'][b],   base[b] = sum_{b' < b} sum_c count[c][b'],
 // i.e. tile-major order with rows in item order.  Rows are scanned in groups of
 // FSCAN_ROWS: k_fscan_part sums each group's columns, k_fscan_base scans the group sums
@@ -1060,10 +1060,14 @@ void launch_step_finish(hipStream_t s, const RunDesc* runs, RunState* st, int nr
 template <typename R>
 void launch_push(hipStream_t s, const PushArgs<R>& a) {
   if (a.nitems <= 0) return;
-  if (a.mode == 0)
-    k_push<R, 0><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
+  if (a.mode != 0)
+    k_push<R, 1, 0><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
+  else if (a.sort_mode == 1)
+    k_push<R, 0, 1><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
+  else if (a.sort_mode == 2)
+    k_push<R, 0, 2><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
   else
-    k_push<R, 1><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
+    k_push<R, 0, 0><<<a.nitems, PUSH_NT, a.lds_bytes, s>>>(a);
 }
 
 template <typename R>

@@ -61,7 +61,7 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
                            SortBufs sb, const int2* pitems, int npitems, const int2* citems,
                            int ncitems, long long ctotal);
 
-#define FSCAN_ROWS 64  // rows (waves of the push grid) per fused count-table scan item
+#define FSCAN_ROWS 64  // rows (push items) per fused count-table scan item
 #define PUSH_NT 512            // threads per push workgroup
 #define PUSH_NW (PUSH_NT / 64)  // waves per push workgroup
 
@@ -82,7 +82,7 @@ struct PushArgs {
   int mode;                     // 0: gather+push+deposit, 1: deposit only
   int lds_bytes;
   // sort fused into the push (mode 0, tile items): 0 off; 1 count the tiles of the new
-  // positions per wave chunk into ftable; 2 scatter the particles tile-major (by their
+  // positions per item into its row of ftable; 2 scatter the particles tile-major (by their
   // positions at the start of this step) into dst, with their ids
   int sort_mode;
   int* ftable;

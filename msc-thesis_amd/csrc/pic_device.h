@@ -34,9 +34,9 @@ struct RunDesc {
   long long tbase;                   // bucket x chunk count table base (nbuckets*nchunks)
   long long toffbase;                // tile offsets base (nbuckets + 1)
   // sort fused into the push (no collisions, sort cadence >= 2): the push before a sort step
-  // counts, per wave of each push item ("chunk"), the tiles of the particles it leaves
-  // behind; the push of the sort step scatters them tile-major into the other buffer
-  int fchunks;                       // push items x PUSH_NW (0: fused sort off)
+  // counts, per push item ("chunk"), the tiles of the particles it leaves behind; the push
+  // of the sort step scatters them tile-major into the other buffer
+  int fchunks;                       // push items (0: fused sort off)
   int fscan_item_begin, fscan_item_end;
   int schunk;                        // particles per chunk of the separate tile sort
   long long ftbase;                  // base of the run's bucket-major count table
