@@ -40,6 +40,8 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   const int PP = rd.fft_pp;
   const int i0 = it.y;
   const long long g0 = rd.gbase + (long long)i0 * ny;
+  double2* twl = lds + a.row_lds;
+  fft_stage_twiddles(twl, ny, a.tw);
   if (a.rho_in) {
     const double* r = a.rho_in + (g0 - rd.gbase);
     for (int t = threadIdx.x; t < PP * ny; t += PIC_WG) {
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
     }
   }
   __syncthreads();
-  fft_lds_rows(lds, PP, ny, logny, -1, a.tw);
+  fft_lds_rows(lds, PP, ny, logny, -1, twl, logny);
   double2* S = a.S + rd.sbase;
   const int lg = ilog2(2 * PP);
   for (int t = threadIdx.x; t < (ny / 2 + 1) * 2 * PP; t += PIC_WG) {
@@ -88,10 +90,12 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
   double2* A = lds + nx;
   double2* B = lds + 2 * nx;
   double2* C = lds + 3 * nx;
+  double2* twl = lds + (a.want_phi ? 4 : 3) * a.max_nx;
   double2* S = a.S + rd.sbase + (long long)k * nx;
+  fft_stage_twiddles(twl, nx, a.tw);
   for (int i = threadIdx.x; i < nx; i += PIC_WG) col[bitrev(i, lognx)] = S[i];
   __syncthreads();
-  fft_lds(col, nx, lognx, -1, a.tw);
+  fft_lds_rows(col, 1, nx, lognx, -1, twl, lognx);
   if (k == 0 && threadIdx.x == 0 && a.diag && !a.rho_in) {
     const double2 r1 = col[1];
     const long long stp = a.st[it.x].step;
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
     if (a.want_phi) C[br] = ph;
   }
   __syncthreads();
-  fft_lds_rows(A, a.want_phi ? 3 : 2, nx, lognx, +1, a.tw);  // A, B, C are consecutive
+  fft_lds_rows(A, a.want_phi ? 3 : 2, nx, lognx, +1, twl, lognx);  // A, B, C consecutive
   double2* Sx = a.Sx + rd.sbase + (long long)k * nx;
   double2* Sy = a.Sy + rd.sbase + (long long)k * nx;
   for (int i = threadIdx.x; i < nx; i += PIC_WG) {
@@ -143,6 +147,8 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   const int lg = ilog2(nr);
   const double2* SA = (MODE == 0) ? a.Sx + rd.sbase + i0 : a.S + rd.sbase + i0;
   const double2* SB = (MODE == 0) ? a.Sy + rd.sbase + i0 : a.S + rd.sbase + i0 + 1;
+  double2* twl = lds + a.row_lds;
+  fft_stage_twiddles(twl, ny, a.tw);
   for (int t = threadIdx.x; t < nr * ny; t += PIC_WG) {
     const int rr = t & (nr - 1), k = t >> lg;
     const int col = (MODE == 0) ? rr : 2 * rr;  // offset of row slot rr from i0
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
     lds[rr * ny + bitrev(k, logny)] = make_double2(A.x - B.y, A.y + B.x);
   }
   __syncthreads();
-  fft_lds_rows(lds, nr, ny, logny, +1, a.tw);
+  fft_lds_rows(lds, nr, ny, logny, +1, twl, logny);
   const double norm = 1.0 / ((double)nx * ny);
   if (MODE == 1) {
     double* ph = a.phi + rd.gbase;
@@ -250,8 +256,10 @@ inline int grid_for(long long n) {
 }  // namespace
 
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev) {
-  const size_t lrow = (size_t)a.row_lds * sizeof(double2);
-  const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3);
+  // data, then the n/2 twiddles of the transform length
+  const size_t lrow = ((size_t)a.row_lds + a.max_ny / 2) * sizeof(double2);
+  const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3) +
+                      (size_t)(a.max_nx / 2) * sizeof(double2);
   if (ev) (void)hipEventRecord(ev[0], s);
   k_fft_rows<<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);
   if (ev) (void)hipEventRecord(ev[1], s);

@@ -15,6 +15,7 @@
 
 #define PIC_WG 256                   // threads per workgroup of every kernel
 #define PIC_FFT_MAX 2048             // largest grid side
+#define PIC_FFT_LOG 11               // log2(PIC_FFT_MAX)
 #define PIC_TW_LOG 11                // twiddle table: exp(-2 pi i k / 2048), k < 1024
 #define PIC_CHUNK 4096               // particles per chunk item
 #define PIC_DIAG_CAP 65536           // diagnostics ring depth (steps)
