@@ -196,6 +196,11 @@ int Ctx::build_items() {
   }
   if (fsum >= (1ll << 31)) fused = false;
   if (fused) lds_push += fused_lds;
+  // The fused sort keys on positions extrapolated to the middle of the next cycle: the
+  // pushes of one sorted layout gather at 1 .. sort_every and deposit at 2 .. sort_every + 1
+  // steps after the state the keys are taken from.
+  for (int r = 0; r < nruns; ++r)
+    rd[r].sort_lead = fused ? 0.5 * (sort_every + 2) * rd[r].dt : 0.0;
   auto up = [&](auto& vec, auto** dptr) -> int {
     using T = typename std::remove_reference<decltype(vec)>::type::value_type;
     if (*dptr) (void)hipFree(*dptr);

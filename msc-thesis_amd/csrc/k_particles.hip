@@ -371,6 +371,19 @@ __device__ __forceinline__ int bucket_of(R x, R y, const Geo<R>& g, const RunDes
   return (cx >> sh) * rd.tiles_y + (cy >> sh);
 }
 
+// Key of the sort fused into the push: the tile of the position extrapolated by
+// rd.sort_lead (half the span of the steps until the next sort).  The particles of a tile
+// range then stay within about half a cycle's drift of their tile, instead of drifting a
+// whole cycle away from it, so far fewer of them leave the tile's LDS windows (gathers and
+// deposits through HBM).  Any key gives the same trajectories; both passes of one sort
+// evaluate it on the same stored values.
+template <typename R>
+__device__ __forceinline__ int bucket_lead(R x, R y, R vx, R vy, const Geo<R>& g,
+                                           const RunDesc& rd) {
+  const R t = (R)rd.sort_lead;
+  return bucket_of<R>(wrap_pos(fmar(t, vx, x), g.Lx), wrap_pos(fmar(t, vy, y), g.Ly), g, rd);
+}
+
 template <typename R>
 __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict__ runs,
                                                       const RunState* __restrict__ st,

@@ -43,6 +43,7 @@ struct RunDesc {
   long long ftbase;                  // base of the run's bucket-major count table
   double lx, ly, dx, dy;
   double dt, qm, qp;
+  double sort_lead;                  // fused sort: keys are tiles of x + sort_lead * v
   double rho_scale;                  // charge per area of one fixed-point unit
   long long pbase;                   // particle base
   long long gbase;                   // node base (nx*ny)
