@@ -196,11 +196,15 @@ int Ctx::build_items() {
   }
   if (fsum >= (1ll << 31)) fused = false;
   if (fused) lds_push += fused_lds;
-  // The fused sort keys on positions extrapolated to the middle of the next cycle: the
+  // Tile sorts key on positions extrapolated to the middle of the next cycle.  Fused: the
   // pushes of one sorted layout gather at 1 .. sort_every and deposit at 2 .. sort_every + 1
-  // steps after the state the keys are taken from.
+  // steps after the state the keys are taken from; separate (at a step's start): 0 ..
+  // sort_every - 1 and 1 .. sort_every.  Not in runs with collisions (their cell lists and
+  // births use the tiles of the positions themselves).
   for (int r = 0; r < nruns; ++r)
-    rd[r].sort_lead = fused ? 0.5 * (sort_every + 2) * rd[r].dt : 0.0;
+    rd[r].sort_lead = cfg[r].collide != PIC_COLLIDE_NONE ? 0.0
+                      : fused ? 0.5 * (sort_every + 2) * rd[r].dt
+                              : 0.5 * sort_every * rd[r].dt;
   auto up = [&](auto& vec, auto** dptr) -> int {
     using T = typename std::remove_reference<decltype(vec)>::type::value_type;
     if (*dptr) (void)hipFree(*dptr);
@@ -1045,7 +1049,7 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
   for (auto& c : x.cfg) mn = std::min(mn, std::min(c.nx, c.ny));
   double best = 1e30;
   int bt = 0, bh = 0, bs = 0;
-  const int tiles[3] = {8, 16, 32}, halos[2] = {8, 16}, cad[4] = {1, 2, 4, 8};
+  const int tiles[3] = {8, 16, 32}, halos[2] = {8, 16}, cad[5] = {1, 2, 4, 6, 8};
   // splits: 0 = sized by particles per tile (the only choice for ragged batches)
   const int splitc[3] = {0, 1, 2};
   int rc = PIC_OK;

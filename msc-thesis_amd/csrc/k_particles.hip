@@ -371,15 +371,17 @@ __device__ __forceinline__ int bucket_of(R x, R y, const Geo<R>& g, const RunDes
   return (cx >> sh) * rd.tiles_y + (cy >> sh);
 }
 
-// Key of the sort fused into the push: the tile of the position extrapolated by
-// rd.sort_lead (half the span of the steps until the next sort).  The particles of a tile
-// range then stay within about half a cycle's drift of their tile, instead of drifting a
-// whole cycle away from it, so far fewer of them leave the tile's LDS windows (gathers and
-// deposits through HBM).  Any key gives the same trajectories; both passes of one sort
-// evaluate it on the same stored values.
+// Key of the tile sorts: the tile of the position extrapolated by rd.sort_lead (about half
+// the span of the steps until the next sort; 0 in runs with collisions, whose cell lists
+// need the tiles of the positions themselves).  The particles of a tile range then stay
+// within about half a cycle's drift of their tile, instead of drifting a whole cycle away
+// from it, so far fewer of them leave the tile's LDS windows (gathers and deposits through
+// HBM).  Any key gives the same trajectories; both passes of one sort evaluate it on the
+// same stored values.
 template <typename R>
 __device__ __forceinline__ int bucket_lead(R x, R y, R vx, R vy, const Geo<R>& g,
                                            const RunDesc& rd) {
+  if (rd.sort_lead == 0.0) return bucket_of<R>(x, y, g, rd);
   const R t = (R)rd.sort_lead;
   return bucket_of<R>(wrap_pos(fmar(t, vx, x), g.Lx), wrap_pos(fmar(t, vy, y), g.Ly), g, rd);
 }
@@ -389,6 +391,8 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
                                                       const RunState* __restrict__ st,
                                                       const R* __restrict__ x,
                                                       const R* __restrict__ y,
+                                                      const R* __restrict__ vx,
+                                                      const R* __restrict__ vy,
                                                       int* __restrict__ table,
                                                       const int2* __restrict__ items) {
   extern __shared__ int h[];
@@ -403,16 +407,20 @@ __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict_
   const int p1 = min(it.y + rd.schunk, np);
   // four particles per thread per pass, loads first
   for (int q = it.y + threadIdx.x; q < p1; q += 4 * PIC_WG) {
-    R xs[4], ys[4];
+    R xs[4], ys[4], vxs[4], vys[4];
+    const bool lead = rd.sort_lead != 0.0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int p = q + u * PIC_WG;
       xs[u] = p < p1 ? x[rd.pbase + p] : (R)0;
       ys[u] = p < p1 ? y[rd.pbase + p] : (R)0;
+      vxs[u] = (lead && p < p1) ? vx[rd.pbase + p] : (R)0;
+      vys[u] = (lead && p < p1) ? vy[rd.pbase + p] : (R)0;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (q + u * PIC_WG < p1) atomicAdd(&h[bucket_of<R>(xs[u], ys[u], g, rd)], 1);
+      if (q + u * PIC_WG < p1)
+        atomicAdd(&h[bucket_lead<R>(xs[u], ys[u], vxs[u], vys[u], g, rd)], 1);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < nb; b += PIC_WG)
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {
       const int p = q0 + 64 * u + lane;
-      key[u] = (p < p1) ? bucket_of<R>(vx[u], vy[u], g, rd) : -1;
+      key[u] = (p < p1) ? bucket_lead<R>(vx[u], vy[u], vvx[u], vvy[u], g, rd) : -1;
     }
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {
@@ -998,7 +1006,8 @@ void launch_tile_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nrun
                       PartBufs<R> src, PartBufs<R> dst, SortBufs sb, const int2* pitems,
                       int npitems) {
   const size_t lds = sizeof(int) * (size_t)sb.max_buckets;
-  k_tile_hist<R><<<npitems, PIC_WG, lds, s>>>(runs, st, src.x, src.y, sb.table, pitems);
+  k_tile_hist<R><<<npitems, PIC_WG, lds, s>>>(runs, st, src.x, src.y, src.vx, src.vy, sb.table,
+                                               pitems);
   k_tscan_local<<<sb.ntscan_items, PIC_WG, 0, s>>>(runs, sb.table, sb.tchunk_sums,
                                                    sb.tscan_items);
   k_tscan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, sb.tchunk_sums);
