@@ -46,15 +46,19 @@ def parse():
 
 
 def read_traffic():
-    """HBM bytes per push_deposit launch measured with rocprofv3 --pmc (see the file)."""
+    """HBM bytes per push_deposit launch measured with rocprofv3 --pmc (see the file), and
+    the tuning "tile,halo,splits,sort_every" it was measured at."""
+    traffic, tuning = None, None
     try:
         with open(PMC_FILE) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
-                    return float(line.split("=")[1])
+                    traffic = float(line.split("=")[1])
+                elif line.startswith("traffic_tuning"):
+                    tuning = line.split("=")[1].strip()
     except OSError:
         pass
-    return None
+    return traffic, tuning
 
 
 def cpu_baseline(parts, cfg, nsteps):
@@ -180,7 +184,11 @@ def main():
         alg_bytes += 8.0 * n / tune["sort_every"]
     push_ms = stages["push_deposit"]
     achieved = alg_bytes / (push_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    # the measured traffic belongs to one tuning: reported only when the bench ran that one
+    traffic, traffic_tuning = read_traffic()
+    if traffic_tuning != "%d,%d,%d,%d" % (tune["tile"], tune["halo"], tune["splits"],
+                                          tune["sort_every"]):
+        traffic = None
     line = {
         "metric": METRIC,
         "value": value,
@@ -200,7 +208,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_push (fused gather+push+deposit)",
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": push_ms},
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": push_ms,
+                     "traffic_tuning": traffic_tuning},
         "stages_ms": stages,
         "algorithmic_roofline_frac": value * (40.0 + 72.0 * G / n) / (HBM_PEAK_GBS * 1e9),
     }
