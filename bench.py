@@ -46,9 +46,9 @@ def parse():
 
 
 def read_traffic():
-    """HBM bytes per push_deposit launch measured with rocprofv3 --pmc (see the file), and
-    the tuning "tile,halo,splits,sort_every" it was measured at."""
-    traffic, tuning = None, None
+    """HBM bytes per push_deposit launch measured with rocprofv3 --pmc (see the file), by
+    the tuning "tile,halo,splits,sort_every" they were measured at."""
+    table, traffic, tuning = {}, None, None
     try:
         with open(PMC_FILE) as f:
             for line in f:
@@ -56,9 +56,14 @@ def read_traffic():
                     traffic = float(line.split("=")[1])
                 elif line.startswith("traffic_tuning"):
                     tuning = line.split("=")[1].strip()
+                elif line.startswith("traffic_at "):
+                    key, val = line[len("traffic_at "):].split("=")
+                    table[key.strip()] = float(val)
     except OSError:
         pass
-    return traffic, tuning
+    if traffic is not None and tuning:
+        table[tuning] = traffic
+    return table
 
 
 def cpu_baseline(parts, cfg, nsteps):
@@ -184,11 +189,10 @@ def main():
         alg_bytes += 8.0 * n / tune["sort_every"]
     push_ms = stages["push_deposit"]
     achieved = alg_bytes / (push_ms * 1e-3) / 1e9
-    # the measured traffic belongs to one tuning: reported only when the bench ran that one
-    traffic, traffic_tuning = read_traffic()
-    if traffic_tuning != "%d,%d,%d,%d" % (tune["tile"], tune["halo"], tune["splits"],
-                                          tune["sort_every"]):
-        traffic = None
+    # the measured traffic belongs to a tuning: reported only for one that was measured
+    traffic_tuning = "%d,%d,%d,%d" % (tune["tile"], tune["halo"], tune["splits"],
+                                      tune["sort_every"])
+    traffic = read_traffic().get(traffic_tuning)
     line = {
         "metric": METRIC,
         "value": value,

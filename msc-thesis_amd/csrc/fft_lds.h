@@ -87,9 +87,11 @@ __device__ __forceinline__ void fft_lds(double2* a, int n, int logn, int sign,
   fft_lds_rows(a, 1, n, logn, sign, tw);
 }
 
-// sin(2 pi m / n) from the twiddle table
-__device__ __forceinline__ double sin2pi_frac(int m, int n, const double2* __restrict__ tw) {
-  const int st = PIC_FFT_MAX / n;
+// sin(2 pi m / n) from a twiddle table of 2^logtw points (the global one by default, or the
+// n-point copy in LDS with logtw = log2 n)
+__device__ __forceinline__ double sin2pi_frac(int m, int n, const double2* __restrict__ tw,
+                                              int logtw = PIC_FFT_LOG) {
+  const int st = (1 << logtw) / n;
   if (m == 0 || 2 * m == n) return 0.0;
   if (2 * m < n) return -tw[m * st].y;
   return tw[(n - m) * st].y;

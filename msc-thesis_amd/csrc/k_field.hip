@@ -105,7 +105,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
   const double twopi = 6.283185307179586476925;
   const int kk = (2 * k < ny) ? k : k - ny;
   const double ky = twopi * kk / rd.ly;
-  const double sy = sin2pi_frac(k, ny, a.tw) / rd.dy;
+  const double sy = sin2pi_frac(k, ny, a.tw) / rd.dy;  // one value per workgroup
   for (int m = threadIdx.x; m < nx; m += PIC_WG) {
     const int mm = (2 * m < nx) ? m : m - nx;
     const double kx = twopi * mm / rd.lx;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
     const double2 r = col[m];
     double2 ph = make_double2(0.0, 0.0);
     if (m != 0 || k != 0) ph = make_double2(r.x / k2, r.y / k2);
-    const double sx = sin2pi_frac(m, nx, a.tw) / rd.dx;
+    const double sx = sin2pi_frac(m, nx, twl, lognx) / rd.dx;
     const int br = bitrev(m, lognx);
     A[br] = make_double2(sx * ph.y, -sx * ph.x);
     B[br] = make_double2(sy * ph.y, -sy * ph.x);
