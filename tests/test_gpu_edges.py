@@ -162,3 +162,26 @@ def test_batch_with_a_one_particle_run(oracle):
     for s, g in zip(sep, got):
         for u, v in zip(s, g):
             assert np.array_equal(u, v)
+
+
+def test_fast_particles_in_the_extrapolated_sort_keys(oracle):
+    """The tile sorts key on x + lead * v (lead up to (sort_every + 2) / 2 steps): with
+    particles crossing the domain several times per step, the extrapolated positions wrap
+    many times over.  The order may change, the trajectories may not."""
+    rng = np.random.default_rng(6)
+    n = 20_000
+    L = FOUR_PI
+    x = rng.uniform(0, L, n)
+    y = rng.uniform(0, L, n)
+    vx = rng.normal(0, 1.0, n)
+    vy = rng.normal(0, 1.0, n)
+    vx[::7] *= 800.0   # ~40 box lengths per step for a seventh of the particles
+    vy[::5] *= 800.0
+    cfg = config(32, 32, L, L, n)
+    base, _ = run_engine(cfg, (x, y, vx, vy), 9, dict(tile=8, halo=2, sort_every=1))
+    for tuning in (dict(tile=8, halo=2, sort_every=4), dict(tile=16, halo=4, sort_every=8)):
+        got, _ = run_engine(cfg, (x, y, vx, vy), 9, tuning)
+        for u, v in zip(base, got):
+            assert np.array_equal(u, v)
+    ref, _ = run_mirror(oracle, cfg, (x, y, vx, vy), 9)
+    assert_close_to_mirror(base, ref, L, L, atol=2e-3)
