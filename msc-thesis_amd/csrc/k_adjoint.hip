@@ -494,7 +494,7 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_rows_fwd(AdjGrid g, double* __re
                                                          double2* __restrict__ A,
                                                          const double2* __restrict__ tw) {
   extern __shared__ double2 s[];
-  const int i = blockIdx.x, ny = g.ny, G = g.nx * g.ny;
+  const int i = xcd_item(blockIdx.x, gridDim.x), ny = g.ny, G = g.nx * g.ny;
   for (int j = threadIdx.x; j < ny; j += PIC_WG) {
     s[bitrev(j, g.logny)] = make_double2(lE[i * ny + j], lE[G + i * ny + j]);
     lE[i * ny + j] = 0.0;
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(PIC_WG) void k_adj_rows_inv(AdjGrid g, const double
                                                          double* __restrict__ lrho,
                                                          const double2* __restrict__ tw) {
   extern __shared__ double2 s[];
-  const int i = blockIdx.x, ny = g.ny;
+  const int i = xcd_item(blockIdx.x, gridDim.x), ny = g.ny;
   for (int k = threadIdx.x; k < ny; k += PIC_WG) s[bitrev(k, g.logny)] = A[(long long)k * g.nx + i];
   __syncthreads();
   fft_lds(s, ny, g.logny, +1, tw);

@@ -34,7 +34,7 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 // consecutive threads on consecutive rows.
 __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   extern __shared__ double2 lds[];
-  const int2 it = a.row_pair_items[blockIdx.x];
+  const int2 it = a.row_pair_items[xcd_item(blockIdx.x, gridDim.x)];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
   const int PP = rd.fft_pp;
@@ -138,7 +138,8 @@ template <typename R, int MODE>
 __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   extern __shared__ double2 lds[];
   __shared__ double red[4];
-  const int2 it = (MODE == 0) ? a.row_items[blockIdx.x] : a.row_pair_items[blockIdx.x];
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int2 it = (MODE == 0) ? a.row_items[item] : a.row_pair_items[item];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
   const int i0 = it.y;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   // field-energy partial of this row group (k_step_finish sums them per run in row order)
   if (MODE != 0 || !a.partials) return;
   const double s = block_sum(en, red);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+  if (threadIdx.x == 0) a.partials[item] = s;
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_op_efield(const double* __restrict__ phi,

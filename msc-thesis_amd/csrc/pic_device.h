@@ -110,6 +110,15 @@ __device__ __forceinline__ R fmar(R a, R b, R c) {
 // any int v, one AND
 __device__ __forceinline__ int pwrap(int v, int n) { return v & (n - 1); }
 
+// Item handled by workgroup b of an n-workgroup launch, such that each XCD gets a
+// contiguous range of items: the hardware dispatches workgroup b to XCD b % 8, and items
+// next to each other share cache lines (the transposed spectra), which each XCD's own L2
+// would otherwise fetch separately.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int x = b & 7, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 template <typename R>
 __device__ __forceinline__ R wrap_pos(R x, R L) {
   if (x >= L) x -= L;
