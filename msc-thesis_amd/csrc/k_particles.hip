@@ -1045,30 +1045,44 @@ __global__ __launch_bounds__(PIC_WG) void k_step_finish(const RunDesc* __restric
                                                         const double* __restrict__ ppush,
                                                         const double* __restrict__ prows,
                                                         double* __restrict__ diag) {
-  __shared__ double red[4];
+  __shared__ double red[4][4];
   const int run = blockIdx.x;
-  const RunDesc rd = runs[run];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int i = rd.push_item_begin + threadIdx.x; i < rd.push_item_end; i += PIC_WG) {
-    s0 += ppush[3 * (long long)i + 0];
-    s1 += ppush[3 * (long long)i + 1];
-    s2 += ppush[3 * (long long)i + 2];
-  }
-  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) s3 += prows[q];
-  s0 = block_sum(s0, red);
-  s1 = block_sum(s1, red);
-  s2 = block_sum(s2, red);
-  s3 = block_sum(s3, red);
+  // the step counter and particle count do not depend on the sums: loaded first, so their
+  // latency overlaps the partials'
+  long long stp = 0;
+  int np = 0;
   if (threadIdx.x == 0) {
+    stp = st[run].step;
+    np = st[run].np;
+  }
+  const RunDesc rd = runs[run];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = rd.push_item_begin + threadIdx.x; i < rd.push_item_end; i += PIC_WG) {
+    v[0] += ppush[3 * (long long)i + 0];
+    v[1] += ppush[3 * (long long)i + 1];
+    v[2] += ppush[3 * (long long)i + 2];
+  }
+  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) v[3] += prows[q];
+  // the four workgroup sums in one pass (each: wave sums, then the four waves in order)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double s0 = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    const double s1 = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+    const double s2 = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+    const double s3 = ((red[3][0] + red[3][1]) + red[3][2]) + red[3][3];
     RunState* sr = st + run;
-    const long long stp = sr->step;
     const double mp = rd.qp / rd.qm;
     double* row = diag + (rd.dbase + (stp % PIC_DIAG_CAP)) * 8;
     row[0] = 0.5 * s3 * rd.dx * rd.dy;
     row[1] = 0.5 * mp * s0;
     row[2] = mp * s1;
     row[3] = mp * s2;
-    row[5] = (double)sr->np;
+    row[5] = (double)np;
     sr->step = stp + 1;
     sr->vsync = 0;
   }
