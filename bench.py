@@ -6,7 +6,8 @@ dt = 0.05.  One "step" = field solve + fused gather/push/deposit (+ the amortise
 sort).  Inputs are resident in HBM when the timed region starts.
 
 Prints ONE JSON line.  Extra keys: roofline (dominant kernel, HIP-event timed), cpu_baseline
-(the CPU oracle on a bounded sample), stages (ms per step per stage), batched_vs_sequential
+(the CPU oracle on a bounded sample, one thread) and cpu_baseline_all_cores (the same on
+every host thread the process may use), stages (ms per step per stage), batched_vs_sequential
 (48-run ragged ensemble) and adjoint_over_forward (128^2 x 32 ppc, 100 steps).
 """
 import argparse
@@ -66,22 +67,40 @@ def read_traffic():
     return table
 
 
-def cpu_baseline(parts, cfg, nsteps):
-    """The oracle's FP32-mirror step (one thread) on the same particles, bounded sample."""
+def cpu_threads():
+    """Host threads for the all-cores baseline: OMP_NUM_THREADS if set, else the CPUs this
+    process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(parts, cfg, nsteps, nthreads=1):
+    """The oracle's FP32-mirror step on the same particles, bounded sample: one thread (the
+    reference's serial loop), or nthreads OpenMP threads (or_run_f32_threads, bit-identical
+    particles)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import load
     o = load()
     n = parts[0].size
     arrs = [a.astype(np.float32) for a in parts]
+    qp = -(cfg.lx * cfg.ly) / n
     t0 = time.perf_counter()
-    o.run_f32(*arrs, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm, -(cfg.lx * cfg.ly) / n,
-              nsteps)
+    if nthreads == 1:
+        o.run_f32(*arrs, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm, qp, nsteps)
+        fn = "or_run_f32, 1 thread"
+    else:
+        o.run_f32_threads(*arrs, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm, qp, nsteps,
+                          nthreads)
+        fn = f"or_run_f32_threads, {nthreads} OpenMP threads"
     dt = time.perf_counter() - t0
-    return {"value": n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+    return {"value": n * nsteps / dt, "unit": "particle-steps/s", "cores": nthreads,
+            "kind": "port",
             "sample": f"{nsteps} steps of the same {cfg.nx}x{cfg.ny} x {n} particle Landau "
-                      f"workload through the FP32-mirror oracle (or_run_f32, 1 thread; the "
-                      f"time includes its initial deposit and final field solve)",
+                      f"workload through the FP32-mirror oracle ({fn}; the time includes "
+                      f"its initial deposit and final field solve)",
             "seconds": dt}
 
 
@@ -219,6 +238,9 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(parts, cfg, 40)
+        nt = cpu_threads()
+        if nt > 1:
+            line["cpu_baseline_all_cores"] = cpu_baseline(parts, cfg, 80, nt)
     if not args.no_batch:
         try:
             line["batched_vs_sequential"] = batch_ratio()

@@ -19,6 +19,9 @@
 // x in [0, L)); the MCC and DSMC physics are the analytic cross-section null-collision
 // and NTC hard-sphere models, which have no reference code.
 #include "oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include <algorithm>
 #include <cmath>
@@ -346,6 +349,79 @@ void fixed_weights(R x, R y, R idx, R idy, int nx, int ny, int* cx, int* cy, int
   w[1] = ix1 * iy0;
   w[2] = ix0 * iy1;
   w[3] = ix1 * iy1;
+}
+
+// final half kick of a run (mirror arithmetic, no drift)
+void half_kick_f32(int64_t n, const float* x, const float* y, float* vx, float* vy,
+                   const float* exf, const float* eyf, int nx, int ny, double lx, double ly,
+                   double dt, double qm) {
+  const float idx = (float)(1.0 / (lx / nx)), idy = (float)(1.0 / (ly / ny));
+  const float hk = (float)(0.5 * dt * qm);
+  for (int64_t p = 0; p < n; ++p) {
+    int cx, cy;
+    float fx, fy;
+    locate_w(x[p], idx, nx, &cx, &fx);
+    locate_w(y[p], idy, ny, &cy, &fy);
+    int ix1 = (int)std::rint(fx * 262144.0f), iy1 = (int)std::rint(fy * 262144.0f);
+    float gx1 = (float)ix1 * (1.0f / 262144.0f), gy1 = (float)iy1 * (1.0f / 262144.0f);
+    float gx0 = 1.0f - gx1, gy0 = 1.0f - gy1;
+    float g00 = gx0 * gy0, g10 = gx1 * gy0, g01 = gx0 * gy1, g11 = gx1 * gy1;
+    int cxp = wrapi(cx + 1, nx), cyp = wrapi(cy + 1, ny);
+    size_t n00 = (size_t)cx * ny + cy, n10 = (size_t)cxp * ny + cy,
+           n01 = (size_t)cx * ny + cyp, n11 = (size_t)cxp * ny + cyp;
+    float epx = g00 * exf[n00];
+    epx = std::fma(g10, exf[n10], epx);
+    epx = std::fma(g01, exf[n01], epx);
+    epx = std::fma(g11, exf[n11], epx);
+    float epy = g00 * eyf[n00];
+    epy = std::fma(g10, eyf[n10], epy);
+    epy = std::fma(g01, eyf[n01], epy);
+    epy = std::fma(g11, eyf[n11], epy);
+    vx[p] = std::fma(hk, epx, vx[p]);
+    vy[p] = std::fma(hk, epy, vy[p]);
+  }
+}
+
+// ---- threaded pieces of the FP32 mirror run (the all-cores CPU baseline).  Each does the
+// serial arithmetic on its share: FFT rows / columns are independent transforms, particles
+// are split in contiguous chunks, and the int64 charge of the chunks is summed per node
+// (integer, so identical to the serial deposit).
+void fft2d_threads(std::vector<cd>& a, int nx, int ny, int sign) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; ++i) fft1(&a[(size_t)i * ny], ny, sign);
+#pragma omp parallel
+  {
+    std::vector<cd> col(nx);
+#pragma omp for schedule(static)
+    for (int j = 0; j < ny; ++j) {
+      for (int i = 0; i < nx; ++i) col[i] = a[(size_t)i * ny + j];
+      fft1(col.data(), nx, sign);
+      for (int i = 0; i < nx; ++i) a[(size_t)i * ny + j] = col[i];
+    }
+  }
+}
+
+void poisson_threads(int nx, int ny, double lx, double ly, const double* rho, double* phi,
+                     double* mode_amp) {
+  std::vector<cd> a((size_t)nx * ny);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)a.size(); ++i) a[i] = cd(rho[i], 0.0);
+  fft2d_threads(a, nx, ny, -1);
+  if (mode_amp) *mode_amp = std::abs(a[(size_t)1 * ny + 0]) / ((double)nx * ny);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nx; ++i) {
+    double kx = kwave(i, nx, lx);
+    for (int j = 0; j < ny; ++j) {
+      double ky = kwave(j, ny, ly);
+      double k2 = kx * kx + ky * ky;
+      size_t id = (size_t)i * ny + j;
+      a[id] = (i == 0 && j == 0) ? cd(0.0, 0.0) : a[id] / k2;
+    }
+  }
+  fft2d_threads(a, nx, ny, +1);
+  double norm = 1.0 / ((double)nx * ny);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)a.size(); ++i) phi[i] = a[i].real() * norm;
 }
 
 }  // namespace
@@ -678,32 +754,7 @@ void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy, int nx, int
     efield_impl(nx, ny, lx, ly, phi.data(), ex.data(), ey.data());
     for (size_t i = 0; i < G; ++i) { exf[i] = (float)ex[i]; eyf[i] = (float)ey[i]; }
     if (s == nsteps) {
-      // final half kick (mirror arithmetic, no drift)
-      const float idx = (float)(1.0 / (lx / nx)), idy = (float)(1.0 / (ly / ny));
-      const float hk = (float)(0.5 * dt * qm);
-      for (int64_t p = 0; p < n; ++p) {
-        int cx, cy;
-        float fx, fy;
-        locate_w(x[p], idx, nx, &cx, &fx);
-        locate_w(y[p], idy, ny, &cy, &fy);
-        int ix1 = (int)std::rint(fx * 262144.0f), iy1 = (int)std::rint(fy * 262144.0f);
-        float gx1 = (float)ix1 * (1.0f / 262144.0f), gy1 = (float)iy1 * (1.0f / 262144.0f);
-        float gx0 = 1.0f - gx1, gy0 = 1.0f - gy1;
-        float g00 = gx0 * gy0, g10 = gx1 * gy0, g01 = gx0 * gy1, g11 = gx1 * gy1;
-        int cxp = wrapi(cx + 1, nx), cyp = wrapi(cy + 1, ny);
-        size_t n00 = (size_t)cx * ny + cy, n10 = (size_t)cxp * ny + cy,
-               n01 = (size_t)cx * ny + cyp, n11 = (size_t)cxp * ny + cyp;
-        float epx = g00 * exf[n00];
-        epx = std::fma(g10, exf[n10], epx);
-        epx = std::fma(g01, exf[n01], epx);
-        epx = std::fma(g11, exf[n11], epx);
-        float epy = g00 * eyf[n00];
-        epy = std::fma(g10, eyf[n10], epy);
-        epy = std::fma(g01, eyf[n01], epy);
-        epy = std::fma(g11, eyf[n11], epy);
-        vx[p] = std::fma(hk, epx, vx[p]);
-        vy[p] = std::fma(hk, epy, vy[p]);
-      }
+      half_kick_f32(n, x, y, vx, vy, exf.data(), eyf.data(), nx, ny, lx, ly, dt, qm);
       break;
     }
     double ke, px, py;
@@ -718,6 +769,75 @@ void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy, int nx, int
       d[4] = amp;
     }
   }
+}
+
+void or_run_f32_threads(int64_t n, float* x, float* y, float* vx, float* vy, int nx, int ny,
+                        double lx, double ly, double dt, double qm, double qp, int nsteps,
+                        int nthreads, double* diag) {
+  const int nt = nthreads > 0 ? nthreads : 1;
+  const size_t G = (size_t)nx * ny;
+  std::vector<int64_t> acc(G), part((size_t)nt * G);
+  std::vector<double> rho(G), phi(G), ex(G), ey(G), sums((size_t)nt * 3);
+  std::vector<float> exf(G), eyf(G);
+  const double mp = qp / qm;
+  auto chunk = [&](int t, int64_t* b, int64_t* e) {
+    *b = n * t / nt;
+    *e = n * (t + 1) / nt;
+  };
+#ifdef _OPENMP
+  const int saved = omp_get_max_threads();
+  omp_set_num_threads(nt);
+#endif
+  for (int s = 0; s <= nsteps; ++s) {
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+      int64_t b, e;
+      chunk(t, &b, &e);
+      or_deposit_fixed_f32(e - b, x + b, y + b, nx, ny, lx, ly, part.data() + (size_t)t * G);
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)G; ++i) {
+      int64_t v = 0;
+      for (int t = 0; t < nt; ++t) v += part[(size_t)t * G + i];
+      acc[i] = v;
+    }
+    or_fixed_to_rho_f32(nx, ny, lx, ly, qp, acc.data(), rho.data());
+    double amp;
+    poisson_threads(nx, ny, lx, ly, rho.data(), phi.data(), &amp);
+    efield_impl(nx, ny, lx, ly, phi.data(), ex.data(), ey.data());
+    for (size_t i = 0; i < G; ++i) { exf[i] = (float)ex[i]; eyf[i] = (float)ey[i]; }
+    if (s == nsteps) {
+#pragma omp parallel for schedule(static, 1)
+      for (int t = 0; t < nt; ++t) {
+        int64_t b, e;
+        chunk(t, &b, &e);
+        half_kick_f32(e - b, x + b, y + b, vx + b, vy + b, exf.data(), eyf.data(), nx, ny, lx,
+                      ly, dt, qm);
+      }
+      break;
+    }
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+      int64_t b, e;
+      chunk(t, &b, &e);
+      push_mirror<float, 18>(e - b, x + b, y + b, vx + b, vy + b, exf.data(), eyf.data(), nx,
+                             ny, lx, ly, dt, qm, s == 0 ? 0.5 : 1.0, &sums[3 * t],
+                             &sums[3 * t + 1], &sums[3 * t + 2]);
+    }
+    if (diag) {
+      double ke = 0, px = 0, py = 0;
+      for (int t = 0; t < nt; ++t) { ke += sums[3 * t]; px += sums[3 * t + 1]; py += sums[3 * t + 2]; }
+      double* d = diag + (size_t)s * 5;
+      d[0] = field_energy(nx, ny, lx, ly, ex.data(), ey.data());
+      d[1] = 0.5 * mp * ke;
+      d[2] = mp * px;
+      d[3] = mp * py;
+      d[4] = amp;
+    }
+  }
+#ifdef _OPENMP
+  omp_set_num_threads(saved);
+#endif
 }
 
 // ------------------------------------------------------------ collisions

@@ -131,6 +131,13 @@ void or_run_f64(int64_t n, double* x, double* y, double* vx, double* vy,
 void or_run_f32(int64_t n, float* x, float* y, float* vx, float* vy,
                 int nx, int ny, double lx, double ly, double dt, double qm,
                 double qp, int nsteps, double* diag);
+/* or_run_f32 on nthreads OpenMP threads (the all-cores CPU baseline): particles in
+ * contiguous chunks, per-thread int64 charge grids summed per node, FFT rows and columns
+ * split across threads.  Positions and velocities are bit-identical to or_run_f32; the
+ * diagnostics' particle sums are added per chunk, so they may differ in the last bits. */
+void or_run_f32_threads(int64_t n, float* x, float* y, float* vx, float* vy,
+                        int nx, int ny, double lx, double ly, double dt, double qm,
+                        double qp, int nsteps, int nthreads, double* diag);
 
 /* ---------------- collisions ---------------- */
 /* Analytic cross sections (m^2) at energy E (eV). */

@@ -48,6 +48,8 @@ SIG = {
     "or_gather_f64": (None, [_I64, _DP, _DP, _DP, _DP, _I, _I, _D, _D, _DP, _DP]),
     "or_run_f64": (None, [_I64, _DP, _DP, _DP, _DP, _I, _I, _D, _D, _D, _D, _D, _I, _DP]),
     "or_run_f32": (None, [_I64, _FP, _FP, _FP, _FP, _I, _I, _D, _D, _D, _D, _D, _I, _DP]),
+    "or_run_f32_threads": (None, [_I64, _FP, _FP, _FP, _FP, _I, _I, _D, _D, _D, _D, _D, _I,
+                                  _I, _DP]),
     "or_mcc_cross_sections": (None, [_D, _DP, _DP, _DP]),
     "or_mcc_step_f32": (_I64, [_I64, _I64, _FP, _FP, _FP, _FP, _IP, _I64, _D, _D, _D, _D, _D,
                                ctypes.c_uint64, _I64, _LP]),
@@ -184,6 +186,12 @@ class Oracle:
         diag = np.zeros((nsteps, 5))
         self.lib.or_run_f32(x.size, self.p(x), self.p(y), self.p(vx), self.p(vy), nx, ny, lx,
                             ly, dt, qm, qp, nsteps, self.p(diag))
+        return diag
+
+    def run_f32_threads(self, x, y, vx, vy, nx, ny, lx, ly, dt, qm, qp, nsteps, nthreads):
+        diag = np.zeros((nsteps, 5))
+        self.lib.or_run_f32_threads(x.size, self.p(x), self.p(y), self.p(vx), self.p(vy), nx,
+                                    ny, lx, ly, dt, qm, qp, nsteps, nthreads, self.p(diag))
         return diag
 
     def adjoint_mt(self, lex, ley, lx, ly):
