@@ -163,11 +163,12 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms_per_step);
 /* Run nsteps eagerly with hipEvents around every kernel; out_ms[PIC_NSTAGES] receives the
  * mean duration per step of each stage (ms). */
 #define PIC_NSTAGES 6
-#define PIC_STAGE_PUSH_DEPOSIT 0      /* fused gather + push + deposit */
+#define PIC_STAGE_PUSH_DEPOSIT 0      /* fused gather + push + deposit (the k_push launch) */
 #define PIC_STAGE_FFT_ROWS     1      /* rho -> row spectra */
 #define PIC_STAGE_FFT_COLS     2      /* column FFT + Green's function + inverse columns */
 #define PIC_STAGE_FFT_FIELD    3      /* inverse rows -> E (+ field energy) */
-#define PIC_STAGE_SORT         4      /* cell sort (amortised per step) */
+#define PIC_STAGE_SORT         4      /* sorts (amortised per step) and the step's
+                                         diagnostics reduction */
 #define PIC_STAGE_COLLIDE      5      /* MCC or DSMC */
 int pic_stage_times(pic_ctx* ctx, int nsteps, double* out_ms);
 

@@ -474,6 +474,7 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev) {
     else if (steps_since_sort == sort_every - 1) smode = 2;
   }
   launch_push_stage(0, smode);
+  if (ev) (void)hipEventRecord(ev[7], stream);  // the push stage is the k_push launch alone
   launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
   if (smode == 1) {
     sb.toff = toffs[1 - cur];
@@ -1103,14 +1104,17 @@ int pic_stage_times(pic_ctx* ctx, int nsteps, double* out) {
     x.launch_one_step(true, x.ev);
     PIC_HIP(hipEventRecord(x.ev[6], x.stream));
     PIC_HIP(hipEventSynchronize(x.ev[6]));
-    float ms[6];
-    for (int k = 0; k < 6; ++k) PIC_HIP(hipEventElapsedTime(&ms[k], x.ev[k], x.ev[k + 1]));
-    acc[PIC_STAGE_FFT_ROWS] += ms[0];
-    acc[PIC_STAGE_FFT_COLS] += ms[1];
-    acc[PIC_STAGE_FFT_FIELD] += ms[2];
-    acc[PIC_STAGE_PUSH_DEPOSIT] += ms[3];
-    acc[PIC_STAGE_COLLIDE] += ms[4];
-    acc[PIC_STAGE_SORT] += ms[5];
+    // event order: 0 rows 1 cols 2 field 3 push 7 finish + fused-sort scans 4 collide 5
+    // separate sort 6
+    const int from[7] = {0, 1, 2, 3, 7, 4, 5}, to[7] = {1, 2, 3, 7, 4, 5, 6};
+    const int stage[7] = {PIC_STAGE_FFT_ROWS, PIC_STAGE_FFT_COLS, PIC_STAGE_FFT_FIELD,
+                          PIC_STAGE_PUSH_DEPOSIT, PIC_STAGE_SORT, PIC_STAGE_COLLIDE,
+                          PIC_STAGE_SORT};
+    for (int k = 0; k < 7; ++k) {
+      float ms = 0.f;
+      PIC_HIP(hipEventElapsedTime(&ms, x.ev[from[k]], x.ev[to[k]]));
+      acc[stage[k]] += ms;
+    }
   }
   for (int k = 0; k < PIC_NSTAGES; ++k) out[k] = acc[k] / nsteps;
   return PIC_OK;
