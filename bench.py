@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--stream", action="store_true",
+                    help="run only the streaming reference: an in-place read-modify-write of "
+                         "four 64 MiB FP32 arrays (the plain push's particle state) with the "
+                         "caches flushed before each pass")
     ap.add_argument("--only", choices=["batch", "adjoint"], default=None,
                     help="run only the ensemble comparison or only the adjoint timing "
                          "(for profiling those legs)")
@@ -104,6 +108,36 @@ def cpu_baseline(parts, cfg, nsteps, nthreads=1):
             "seconds": dt}
 
 
+def stream_probe(reps=20, n=1 << 24):
+    """What a plain streaming pass over the push's particle state costs on this device: four
+    in-place x += 1 passes over 16,777,216 FP32 values each (the x, y, vx, vy arrays of the
+    benched workload: 536.9 MB read + written), timed with HIP events, after a 1 GiB fill
+    that evicts them from the L2 and the last-level cache (cold), and repeated without the
+    fill (warm: partly served from the last-level cache)."""
+    import torch
+    arrs = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(4)]
+    flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
+    for mode in ("cold", "warm"):
+        ms = []
+        for r in range(reps):
+            if mode == "cold":
+                flush.fill_(float(r))
+            e0.record()
+            for a in arrs:
+                a.add_(1.0)
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        ms.sort()
+        med = ms[len(ms) // 2]
+        out[mode] = {"median_us": med * 1e3, "min_us": ms[0] * 1e3,
+                     "GB_per_s": 8.0 * n * 4 / (med * 1e-3) / 1e9}
+    out["bytes"] = 8 * n * 4
+    return out
+
+
 def batch_ratio(steps=300):
     """48 ragged runs: one batched context vs the same runs one at a time.  Both sides are
     autotuned first (the batch over tile/halo/sort cadence for all runs at once, each
@@ -163,6 +197,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}))
+        return
+    if args.stream:
+        print(json.dumps({"stream_rmw": stream_probe()}))
         return
     if args.only == "batch":
         print(json.dumps({"batched_vs_sequential": batch_ratio()}))
