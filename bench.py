@@ -8,7 +8,8 @@ sort).  Inputs are resident in HBM when the timed region starts.
 Prints ONE JSON line.  Extra keys: roofline (dominant kernel, HIP-event timed), cpu_baseline
 (the CPU oracle on a bounded sample, one thread) and cpu_baseline_all_cores (the same on
 every host thread the process may use), stages (ms per step per stage), batched_vs_sequential
-(48-run ragged ensemble) and adjoint_over_forward (128^2 x 32 ppc, 100 steps).
+(48-run ragged ensemble), adjoint_over_forward (128^2 x 32 ppc, 100 steps) and mcc_256
+(the collision workload: 256^2, 8.4M electrons, null-collision MCC).
 """
 import argparse
 import json
@@ -33,13 +34,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--no-mcc", action="store_true", help="skip the collision workload")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--stream", action="store_true",
                     help="run only the streaming reference: an in-place read-modify-write of "
                          "four 64 MiB FP32 arrays (the plain push's particle state) with the "
                          "caches flushed before each pass")
-    ap.add_argument("--only", choices=["batch", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["batch", "adjoint", "mcc"], default=None,
                     help="run only the ensemble comparison or only the adjoint timing "
                          "(for profiling those legs)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -138,6 +140,31 @@ def stream_probe(reps=20, n=1 << 24):
     return out
 
 
+def mcc_rate(steps=100, warmup=10):
+    """Config 3: electron-neutral null-collision MCC on 256 x 256 with 8,388,608 electrons
+    (Te = 2 eV, neutrals 1e21 m^-3), the preset's tiling and sort cadence; particle-steps/s
+    of the whole step (field, push, collisions, births, sort) after warm-up steps."""
+    import torch
+    from pic_amd import Simulation, presets
+    cfg, parts, _ = presets.mcc_256()
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(warmup)
+        sim.sync()
+        torch.cuda.synchronize()
+        n0 = sim.num_particles()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        counts = sim.collision_counts()
+        n1 = sim.num_particles()
+    return {"particle_steps_per_s": 0.5 * (n0 + n1) * steps / dt, "ms_per_step": dt * 1e3 / steps,
+            "particles": n1, "steps": steps, "collision_counts": counts,
+            "grid": [cfg.nx, cfg.ny]}
+
+
 def batch_ratio(steps=300):
     """48 ragged runs: one batched context vs the same runs one at a time.  Both sides are
     autotuned first (the batch over tile/halo/sort cadence for all runs at once, each
@@ -203,6 +230,9 @@ def main():
         return
     if args.only == "batch":
         print(json.dumps({"batched_vs_sequential": batch_ratio()}))
+        return
+    if args.only == "mcc":
+        print(json.dumps({"mcc_256": mcc_rate()}))
         return
     if args.only == "adjoint":
         from pic_amd import _lib
@@ -291,6 +321,11 @@ def main():
                                             "f64": time_adjoint(precision=_lib.PIC_FP64)}
         except Exception as e:
             line["adjoint_over_forward"] = {"error": repr(e)}
+    if not args.no_mcc:
+        try:
+            line["mcc_256"] = mcc_rate()
+        except Exception as e:  # reported, not hidden
+            line["mcc_256"] = {"error": repr(e)}
     print(json.dumps(line))
 
 

@@ -237,3 +237,46 @@ def test_engine_dsmc_run():
         x, y, vx, vy = sim.particles()
     assert cnt[0] > 0
     assert np.all((x >= 0) & (x < cfg.lx))
+
+
+@pytest.mark.slow
+def test_engine_mcc_at_full_scale():
+    """Config 3 at its own size: 256 x 256, 8,388,608 electrons at Te = 2 eV.  The first
+    step's counts per collision type lie within 3 sigma of the null-collision expectation
+    from the initial velocities (the field of the uniform plasma barely moves them in one
+    step); over 50 steps every birth is an ionisation and the state stays finite and in
+    the box."""
+    import time
+    cfg, parts, _ = presets.mcc_256()
+    p = mcc_params(cfg)
+    vx = np.asarray(parts[2], np.float32).astype(np.float64)
+    vy = np.asarray(parts[3], np.float32).astype(np.float64)
+    v2 = vx * vx + vy * vy
+    e = p["e_per_v2"] * v2
+    sp = np.sqrt(v2)
+    sel = 1e-19 / (1 + e / 10)
+    sex = np.where(e > 11.5, 2e-21 * (1 - 11.5 / np.maximum(e, 1e-30)), 0)
+    siz = np.where(e > 15.8, 3e-20 * np.log(np.maximum(e, 15.8) / 15.8) / np.maximum(e, 1e-30), 0)
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        n0 = sim.num_particles()
+        sim.step(1)
+        sim.sync()
+        cnt = sim.collision_counts()
+        for k, s in enumerate((sel, sex, siz)):
+            prob = p["p_coll"] * p["nu_dt"] * s * sp / p["numax_dt"]
+            mu, sd = prob.sum(), math.sqrt((prob * (1 - prob)).sum())
+            assert abs(cnt[k] - mu) <= 3 * sd + 1, (k, cnt[k], mu, sd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(49)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        cnt = sim.collision_counts()
+        n1 = sim.num_particles()
+        sim.check_finite()
+        x, y, _, _ = sim.particles()
+    print(f"MCC 256^2: {n0 * 49 / dt:.3e} particle-steps/s over 49 steps (incl. launch), "
+          f"counts {cnt}, births {n1 - n0}")
+    assert n1 - n0 == cnt[2]
+    assert np.all((x >= 0) & (x < cfg.lx)) and np.all((y >= 0) & (y < cfg.ly))
