@@ -280,3 +280,52 @@ def test_engine_mcc_at_full_scale():
           f"counts {cnt}, births {n1 - n0}")
     assert n1 - n0 == cnt[2]
     assert np.all((x >= 0) & (x < cfg.lx)) and np.all((y >= 0) & (y < cfg.ly))
+
+
+@pytest.mark.slow
+def test_engine_dsmc_at_full_scale():
+    """The DSMC workload at its own size (256 x 256, a Gaussian blob of 2,097,152 electrons
+    over a uniform neutralising background, NTC hard-sphere collisions per cell on the
+    cell-sorted state every step).  The blob's field energy is ~2000x the electrons'
+    initial kinetic energy, so it expands and heats them and the collision rate climbs to
+    many per particle per step.  Checked at this size: the whole step runs (collisions
+    accepted, state finite and in the box), and the total energy and the momentum drift
+    no more than in the same run without collisions (elastic collisions conserve both;
+    the per-collision conservation is pinned at operator level)."""
+    import time
+    cfg, parts, _ = presets.dsmc_blob()
+
+    def run(collide):
+        c = _lib.default_config(nx=cfg.nx, ny=cfg.ny, lx=cfg.lx, ly=cfg.ly, dt=cfg.dt,
+                                qm=cfg.qm, precision=cfg.precision, seed=cfg.seed,
+                                collide=collide, dsmc_coeff=cfg.dsmc_coeff,
+                                dsmc_gmax0=cfg.dsmc_gmax0)
+        c.capacity = cfg.capacity
+        with Simulation(c) as sim:
+            sim.set_particles(*parts)
+            n = sim.num_particles()
+            sim.step(1)
+            sim.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.step(39)
+            sim.sync()
+            dt = time.perf_counter() - t0
+            diag = sim.diagnostics(0, 40)
+            cnt = sim.collision_counts()
+            sim.check_finite()
+            x, y, _, _ = sim.particles()
+        e = diag[:, 0] + diag[:, 1]
+        drift = np.abs(e - e[0]).max() / e[0]
+        pmax = np.abs(diag[:, 2:4]).max() / math.sqrt(2.0 * diag[0, 1] * cfg.lx * cfg.ly / n)
+        return n * 39 / dt, cnt, drift, pmax, x, y, n
+
+    rate, cnt, drift, pmax, x, y, n = run(_lib.PIC_COLLIDE_DSMC)
+    _, _, drift0, pmax0, _, _, _ = run(_lib.PIC_COLLIDE_NONE)
+    print(f"DSMC 256^2: {rate:.3e} particle-steps/s (incl. launch), collisions {cnt[0]}, "
+          f"energy drift {drift:.2e} (no collisions {drift0:.2e}), momentum / thermal "
+          f"{pmax:.2e} ({pmax0:.2e})")
+    assert cnt[0] > n // 4, cnt
+    assert drift < 1.1 * drift0 + 1e-4, (drift, drift0)
+    assert pmax < 1.1 * pmax0 + 1e-4, (pmax, pmax0)
+    assert np.all((x >= 0) & (x < cfg.lx)) and np.all((y >= 0) & (y < cfg.ly))
