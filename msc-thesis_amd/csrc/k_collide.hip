@@ -152,13 +152,65 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   }
 }
 
-__global__ __launch_bounds__(PIC_WG) void k_dsmc(DsmcArgs a) {
+#define DSMC_WAVES 4        // waves per workgroup; each takes one cell at a time
+#define DSMC_LDS_MAX 1024    // particles of a cell held in LDS (larger cells stay in HBM)
+
+__device__ __forceinline__ double readlane_f64(double x, int t) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), t);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), t);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// candidate k of cell c: the pair (i, j), i != j, the acceptance uniform and the direction
+// of the post-collision relative velocity
+__device__ __forceinline__ void dsmc_candidate(const DsmcArgs& a, int c, uint32_t s32,
+                                               long long k, int n, int& i, int& j, double& u2,
+                                               double& sn, double& cs) {
+  double u[4];
+  draw4((uint32_t)c, s32, (uint32_t)k, PIC_TAG_DSMC, a.seed, u);
+  i = (int)(u[0] * n);
+  i = i > n - 1 ? n - 1 : i;
+  j = (int)(u[1] * (n - 1));
+  j = j > n - 2 ? n - 2 : j;
+  if (j >= i) ++j;
+  u2 = u[2];
+  sincos2pi_own(u[3], sn, cs);
+}
+
+// NTC pass of one cell: candidates in order k = 0 .. M-1; a candidate with relative speed
+// g raises the running bound gm to g, and is accepted when u2 gm < g: the pair's centre-
+// of-mass velocity is kept and its relative velocity turned to the drawn direction
+__device__ __forceinline__ void dsmc_pair(double vix, double viy, double vjx, double vjy,
+                                          double& gm, double u2, double sn, double cs,
+                                          bool& hit, float2& ni, float2& nj) {
+  const double gx = vix - vjx, gy = viy - vjy;
+  const double g = sqrt(gx * gx + gy * gy);
+  if (g > gm) gm = g;
+  hit = u2 * gm < g;
+  const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
+  const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
+  ni = make_float2((float)(cx + hx), (float)(cy + hy));
+  nj = make_float2((float)(cx - hx), (float)(cy - hy));
+}
+
+// One wave per cell.  The cell's velocities are staged in LDS; the wave draws the random
+// numbers of 64 candidates at once (one per lane), then steps through them in order with
+// wave-uniform values (the pass is sequential: a particle may take part in several pairs),
+// so the serial chain per candidate is LDS reads and arithmetic instead of dependent HBM
+// round trips.  The candidates, their order and every operation are those of the one-cell
+// sequential pass, so the result is bit-identical to it.
+__global__ __launch_bounds__(64 * DSMC_WAVES) void k_dsmc(DsmcArgs a) {
+  __shared__ float2 vs[DSMC_WAVES][DSMC_LDS_MAX];
   __shared__ unsigned int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long step = a.st ? a.st->step - 1 : a.step;
   const uint32_t s32 = (uint32_t)step;
-  for (int c = blockIdx.x * PIC_WG + threadIdx.x; c < a.ncells; c += gridDim.x * PIC_WG) {
+  float2* sv = vs[wv];
+  unsigned int acc = 0;  // the same in every lane
+  for (int c = blockIdx.x * DSMC_WAVES + wv; c < a.ncells; c += gridDim.x * DSMC_WAVES) {
     const int b = a.offsets[c], n = a.offsets[c + 1] - b;
     if (n < 2) continue;
     double gm = (double)a.gmax[c];
@@ -167,34 +219,57 @@ __global__ __launch_bounds__(PIC_WG) void k_dsmc(DsmcArgs a) {
     const double mr = 0.5 * (double)n * (double)(n - 1) * a.coeff * gm;
     const double fl = floor(mr);
     const long long M = (long long)fl + ((u[0] < mr - fl) ? 1 : 0);
-    unsigned int acc = 0;
-    for (long long k = 0; k < M; ++k) {
-      draw4((uint32_t)c, s32, (uint32_t)k, PIC_TAG_DSMC, a.seed, u);
-      int i = (int)(u[0] * n);
-      i = i > n - 1 ? n - 1 : i;
-      int j = (int)(u[1] * (n - 1));
-      j = j > n - 2 ? n - 2 : j;
-      if (j >= i) ++j;
-      const long long pi = b + i, pj = b + j;
-      const double vix = a.vx[pi], viy = a.vy[pi], vjx = a.vx[pj], vjy = a.vy[pj];
-      const double gx = vix - vjx, gy = viy - vjy;
-      const double g = sqrt(gx * gx + gy * gy);
-      if (g > gm) gm = g;
-      if (u[2] * gm < g) {
-        double sn, cs;
-        sincos2pi_own(u[3], sn, cs);
-        const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
-        const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
-        a.vx[pi] = (float)(cx + hx);
-        a.vy[pi] = (float)(cy + hy);
-        a.vx[pj] = (float)(cx - hx);
-        a.vy[pj] = (float)(cy - hy);
-        ++acc;
+    if (n > DSMC_LDS_MAX) {
+      // too many particles for the LDS slot: the same pass on the velocities in HBM
+      if (lane == 0) {
+        for (long long k = 0; k < M; ++k) {
+          int i, j;
+          double u2, sn, cs;
+          dsmc_candidate(a, c, s32, k, n, i, j, u2, sn, cs);
+          const long long pi = b + i, pj = b + j;
+          bool hit;
+          float2 ni, nj;
+          dsmc_pair(a.vx[pi], a.vy[pi], a.vx[pj], a.vy[pj], gm, u2, sn, cs, hit, ni, nj);
+          if (hit) {
+            a.vx[pi] = ni.x; a.vy[pi] = ni.y;
+            a.vx[pj] = nj.x; a.vy[pj] = nj.y;
+            ++acc;
+          }
+        }
+        a.gmax[c] = (float)gm;
+      }
+      continue;
+    }
+    for (int t = lane; t < n; t += 64) sv[t] = make_float2(a.vx[b + t], a.vy[b + t]);
+    for (long long k0 = 0; k0 < M; k0 += 64) {
+      int ci = 0, cj = 1;
+      double cu2 = 0.0, csn = 0.0, ccs = 1.0;
+      if (k0 + lane < M) dsmc_candidate(a, c, s32, k0 + lane, n, ci, cj, cu2, csn, ccs);
+      const int tn = (M - k0 < 64) ? (int)(M - k0) : 64;
+      for (int t = 0; t < tn; ++t) {
+        const int i = __builtin_amdgcn_readlane(ci, t), j = __builtin_amdgcn_readlane(cj, t);
+        const float2 fi = sv[i], fj = sv[j];
+        bool hit;
+        float2 ni, nj;
+        dsmc_pair(fi.x, fi.y, fj.x, fj.y, gm, readlane_f64(cu2, t), readlane_f64(csn, t),
+                  readlane_f64(ccs, t), hit, ni, nj);
+        if (hit) {
+          if (lane == 0) {
+            sv[i] = ni;
+            sv[j] = nj;
+          }
+          ++acc;
+        }
       }
     }
-    a.gmax[c] = (float)gm;
-    if (acc) atomicAdd(&cnt, acc);
+    for (int t = lane; t < n; t += 64) {
+      const float2 v = sv[t];
+      a.vx[b + t] = v.x;
+      a.vy[b + t] = v.y;
+    }
+    if (lane == 0) a.gmax[c] = (float)gm;
   }
+  if (lane == 0 && acc) atomicAdd(&cnt, acc);
   __syncthreads();
   if (threadIdx.x == 0 && cnt && a.ncoll)
     atomicAdd((unsigned long long*)a.ncoll, (unsigned long long)cnt);
@@ -214,7 +289,7 @@ void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scra
 }
 
 void launch_dsmc(hipStream_t s, const DsmcArgs& a) {
-  int blocks = (a.ncells + PIC_WG - 1) / PIC_WG;
-  blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-  k_dsmc<<<blocks, PIC_WG, 0, s>>>(a);
+  int blocks = (a.ncells + DSMC_WAVES - 1) / DSMC_WAVES;
+  blocks = blocks < 1 ? 1 : (blocks > 16384 ? 16384 : blocks);
+  k_dsmc<<<blocks, 64 * DSMC_WAVES, 0, s>>>(a);
 }
