@@ -222,6 +222,19 @@ int pic_op_sort(const int32_t* keys, int64_t n, int ncells, int32_t* perm,
  * deterministic fixed-point accumulation.  thesisCode/collisionless/double/src/pic.cu:99-138 */
 int pic_op_deposit(const void* x, const void* y, int64_t n, int precision,
                    const pic_grid* g, double qp, double* rho, void* stream);
+/* Allocation-free variant of pic_op_deposit (same arguments, same result bit for bit).
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The fixed-point accumulators live in a static device array of
+ * 2 * PIC_DEPOSIT_NOALLOC_MAX_NODES int64 words (16 MiB per device, held until the process
+ * ends); grids with nx*ny above PIC_DEPOSIT_NOALLOC_MAX_NODES are refused with
+ * PIC_ERR_UNSUPPORTED.  Every call on one device shares that array: calls that overlap in
+ * time on one device (other streams, other threads) race and must be serialised by the
+ * caller.  The array is cleared at the start of each call by a kernel (no memset node).
+ * The call synchronises `stream` before returning, like pic_op_deposit. */
+#define PIC_DEPOSIT_NOALLOC_MAX_NODES 1048576
+int pic_op_deposit_noalloc(const void* x, const void* y, int64_t n, int precision,
+                           const pic_grid* g, double qp, double* rho, void* stream);
 /* Spectral Poisson, laplacian(phi) = -rho, mean removed (FP64).
  * thesisCode/collisionless/double/src/pic.cu:147-232 */
 int pic_op_poisson(const double* rho, double* phi, const pic_grid* g, void* stream);

@@ -3,10 +3,13 @@
 // The reference keeps each stage as a file-static kernel configured through
 // __constant__ symbols (thesisCode/collisionless/double/src/pic.cu:13-26, 69-420); these
 // entry points expose the same stages with explicit arguments so each can be dropped into
-// another driver or tested alone.  Temporaries are allocated per call.
+// another driver or tested alone.  Temporaries are allocated per call, except in
+// pic_op_deposit_noalloc (a static device array).
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "engine.h"
@@ -162,6 +165,73 @@ int pic_op_deposit(const void* x, const void* y, int64_t n, int precision, const
   const double dx = g->lx / g->nx, dy = g->ly / g->ny;
   const double scale = qp / (dx * dy) * (precision == PIC_FP64 ? 1.0 : std::ldexp(1.0, -36));
   launch_op_fixed_to_double(s, hi, lo, G, precision == PIC_FP64, scale, rho);
+  PIC_HIP(hipGetLastError());
+  PIC_HIP(hipStreamSynchronize(s));
+  return PIC_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free deposit: the fixed-point accumulators live in a file-scope device
+// array instead of per-call hipMalloc.  Every device that runs the library holds its own
+// copy (16 MiB) until the process ends.  In host code the array's name is only a
+// stand-in symbol: its device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned long long scratch[2 * PIC_DEPOSIT_NOALLOC_MAX_NODES];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[64];  // device address of `scratch` per device ordinal
+
+// Device address of `scratch` on the current device, looked up on the first call there.
+int noalloc_scratch(unsigned long long** out) {
+  int dev = 0;
+  PIC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(PIC_ERR_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PIC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (unsigned long long*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PIC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pic_op_deposit_noalloc(const void* x, const void* y, int64_t n, int precision,
+                           const pic_grid* g, double qp, double* rho, void* stream) {
+  int rc = check_grid(g);
+  if (rc) return rc;
+  if (n < 0 || !rho || (n > 0 && (!x || !y))) return fail(PIC_ERR_ARG, "bad arrays");
+  if (precision != PIC_FP32 && precision != PIC_FP64) return fail(PIC_ERR_ARG, "bad precision");
+  const long long G = (long long)g->nx * g->ny;
+  if (G > PIC_DEPOSIT_NOALLOC_MAX_NODES)
+    return fail(PIC_ERR_UNSUPPORTED, "pic_op_deposit_noalloc: nx*ny = " + std::to_string(G) +
+                                         " exceeds its static scratch of " +
+                                         std::to_string(PIC_DEPOSIT_NOALLOC_MAX_NODES) + " nodes");
+  unsigned long long* hi = nullptr;
+  if ((rc = noalloc_scratch(&hi))) return rc;
+  unsigned long long* lo = hi + PIC_DEPOSIT_NOALLOC_MAX_NODES;
+  hipStream_t s = (hipStream_t)stream;
+  // cleared by a fill kernel (the previous call's sums are still there)
+  launch_fill_u64(s, hi, G);
+  if (precision == PIC_FP64) launch_fill_u64(s, lo, G);
+  if (n > 0) {
+    if (precision == PIC_FP64)
+      launch_op_deposit<double>(s, (const double*)x, (const double*)y, n, g->nx, g->ny, g->lx,
+                                g->ly, hi, lo);
+    else
+      launch_op_deposit<float>(s, (const float*)x, (const float*)y, n, g->nx, g->ny, g->lx, g->ly,
+                               hi, nullptr);
+  }
+  const double dx = g->lx / g->nx, dy = g->ly / g->ny;
+  const double scale = qp / (dx * dy) * (precision == PIC_FP64 ? 1.0 : std::ldexp(1.0, -36));
+  launch_op_fixed_to_double(s, hi, precision == PIC_FP64 ? lo : nullptr, G,
+                            precision == PIC_FP64, scale, rho);
   PIC_HIP(hipGetLastError());
   PIC_HIP(hipStreamSynchronize(s));
   return PIC_OK;

@@ -120,6 +120,7 @@ SIGNATURES = {
     "pic_op_cell_keys": (_I, [_P, _P, _I64, _I, _GRID, _P, _P]),
     "pic_op_sort": (_I, [_P, _I64, _I, _P, _P, _P, _P]),
     "pic_op_deposit": (_I, [_P, _P, _I64, _I, _GRID, _D, _P, _P]),
+    "pic_op_deposit_noalloc": (_I, [_P, _P, _I64, _I, _GRID, _D, _P, _P]),
     "pic_op_poisson": (_I, [_P, _P, _GRID, _P]),
     "pic_op_efield": (_I, [_P, _P, _P, _GRID, _P]),
     "pic_op_gather_push": (_I, [_P, _P, _P, _P, _I64, _I, _P, _P, _GRID, _D, _D, _D, _P, _P]),
