@@ -256,9 +256,14 @@ int pic_op_mcc(float* x, float* y, float* vx, float* vy, int32_t* id, int64_t* n
                int64_t cap, const pic_config* cfg, int64_t step, int64_t* counts,
                void* stream);
 /* One NTC hard-sphere DSMC pass over cell-sorted particles: offsets[ncells+1] delimit the
- * cells, id gives the canonical order inside a cell.  gmax[ncells] is the running
- * per-cell maximum relative speed (read and updated).  ncoll (device int64, may be NULL)
- * accumulates accepted collisions.
+ * cells (row-major cell numbering, as from pic_op_cell_keys + pic_op_sort), and the order
+ * of the particles inside a cell is the canonical one (ascending id, as pic_op_sort leaves
+ * it for arrays in id order); `id` is not read.  A cell of n particles draws
+ * M ~ 1/2 n (n-1) coeff gmax candidate pairs, dealt to rounds of n/2 disjoint pairs under a
+ * pseudo-random slot permutation per round; a candidate with relative speed g is accepted
+ * with probability g / max(gmax, g).  gmax[ncells] is the running per-cell maximum
+ * relative speed (read and updated).  ncoll (device int64, may be NULL) accumulates
+ * accepted collisions.
  * Slot of thesisCode/dsmcCollisions/cellTrackerAdvanced/src/pic.cu:791-864. */
 int pic_op_dsmc(float* vx, float* vy, const int32_t* id, const int32_t* offsets,
                 int ncells, float* gmax, double coeff, uint64_t seed, int64_t step,

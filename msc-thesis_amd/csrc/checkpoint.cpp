@@ -9,12 +9,16 @@
 //                                   the DSMC per-cell maxima
 //   x[n], y[n], vx[n], vy[n]        particle state in id order, in the run's precision;
 //                                   velocities as stored (staggered unless vsync = 1)
-//   gmax[ngmax]                     float, DSMC only
+//   gmax[ngmax]                     float, DSMC only: per-cell maximum relative speed of
+//                                   cell cx*ny + cy (row-major, so a run may restart under
+//                                   another tiling), ngmax = nx*ny + 1
 //   uint64 checksum                 FNV-1a over everything before it
 // A restart continues bit for bit: the charge is redeposited from the positions with the
 // same integer arithmetic, and the random streams are keyed by particle id and step.
 #include <cstdio>
 #include <cstring>
+#include <new>
+#include <string>
 #include <vector>
 
 #include "engine.h"
@@ -24,7 +28,7 @@ using namespace pic;
 namespace {
 
 constexpr char kMagic[8] = {'P', 'I', 'C', 'A', 'M', 'D', 'C', 'K'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: DSMC per-cell maxima in row-major cell order
 
 struct CkptHeader {
   char magic[8];
@@ -160,10 +164,24 @@ int pic_load_checkpoint(pic_ctx* ctx, int run, const char* path) {
   if (!same_run(h.cfg, x.cfg[run]))
     return fail(PIC_ERR_INVALID_CONFIG,
                 "checkpoint grid, domain, dt, qm, precision, collisions or seed differ from the run");
-  const size_t rs = x.fp64 ? 8 : 4;
+  // sizes from the header are checked before anything is allocated or the run touched
   const int64_t n = h.n;
-  std::vector<unsigned char> raw(rs * 4 * (size_t)n);
-  std::vector<float> gm((size_t)h.ngmax);
+  const int64_t cap = x.allocated ? (int64_t)x.rd[run].pcap : (int64_t)0x7fff0000;
+  if (n > cap)
+    return fail(PIC_ERR_OVERFLOW, "checkpoint holds " + std::to_string(n) +
+                                      " particles, more than the run capacity");
+  const int64_t want_gm =
+      x.cfg[run].collide == PIC_COLLIDE_DSMC ? (int64_t)x.cfg[run].nx * x.cfg[run].ny + 1 : 0;
+  if (h.ngmax != want_gm) return fail(PIC_ERR_ARG, "corrupt checkpoint (per-cell table size)");
+  const size_t rs = x.fp64 ? 8 : 4;
+  std::vector<unsigned char> raw;
+  std::vector<float> gm;
+  try {
+    raw.resize(rs * 4 * (size_t)n);
+    gm.resize((size_t)h.ngmax);
+  } catch (const std::bad_alloc&) {
+    return fail(PIC_ERR_OOM, "host memory for the checkpoint");
+  }
   Fnv sum;
   sum.add(&h, sizeof(h));
   if (std::fread(raw.data(), 1, raw.size(), fh.f) != raw.size())
@@ -177,7 +195,12 @@ int pic_load_checkpoint(pic_ctx* ctx, int run, const char* path) {
   uint64_t stored = 0;
   if (std::fread(&stored, sizeof(stored), 1, fh.f) != 1 || stored != sum.h)
     return fail(PIC_ERR_ARG, "checkpoint checksum mismatch");
-  std::vector<double> a(4 * (size_t)n);
+  std::vector<double> a;
+  try {
+    a.resize(4 * (size_t)n);
+  } catch (const std::bad_alloc&) {
+    return fail(PIC_ERR_OOM, "host memory for the checkpoint");
+  }
   for (size_t i = 0; i < a.size(); ++i) {
     if (x.fp64) {
       double v;
@@ -199,8 +222,8 @@ int pic_load_checkpoint(pic_ctx* ctx, int run, const char* path) {
   if (rc) return rc;
   PIC_HIP(hipMemcpy(x.d_coll_counts + 4 * run, h.coll, sizeof(h.coll), hipMemcpyHostToDevice));
   if (h.ngmax) {
+    // per-cell maxima in row-major cell order: independent of the tiling in use
     const RunDesc& d = x.rd[run];
-    if ((size_t)h.ngmax != (size_t)d.nx * d.ny + 1) return fail(PIC_ERR_ARG, "corrupt checkpoint");
     PIC_HIP(hipMemcpy(x.gmax + d.cbase, gm.data(), sizeof(float) * gm.size(), hipMemcpyHostToDevice));
   }
   return PIC_OK;

@@ -278,6 +278,22 @@ int Ctx::alloc() {
     any_mcc |= c.collide == PIC_COLLIDE_MCC;
     any_dsmc |= c.collide == PIC_COLLIDE_DSMC;
   }
+  // device memory the buffers below take (particle double buffers, sort keys / ranks /
+  // permutation / segment scratch, MCC birth buffers, grids): refuse before allocating
+  // anything when the device cannot hold it
+  {
+    long long maxcap = 0;
+    for (auto& c : cfg) maxcap = std::max<long long>(maxcap, c.capacity);
+    const double need = (double)ptotal * (8.0 * rsize + 8 + 28) +
+                        (any_mcc ? 24.0 * maxcap : 0.0) + (double)gsum * (fp64 ? 40 : 24) +
+                        (double)esum * 2 * rsize + 48.0 * ssum + 12.0 * ctotal;
+    size_t free_b = 0, total_b = 0;
+    PIC_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (need > (double)free_b)
+      return fail(PIC_ERR_OOM, "the runs need " + std::to_string((long long)(need / 1e9)) +
+                                   " GB of device memory, " +
+                                   std::to_string((long long)(free_b / 1e9)) + " GB are free");
+  }
   PIC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   for (auto& e : ev) PIC_HIP(hipEventCreate(&e));
   PIC_HIP(dalloc(&d_runs, nruns));
@@ -441,6 +457,7 @@ int Ctx::launch_collide_stage() {
       a.counts = d_coll_counts + 4 * r;
       a.birth_count = birth_count; a.birth_parent = birth_parent; a.birth_buf = birth_buf;
       a.from_state_step = 1;
+      a.diag = diag + d.dbase * 8;
       launch_mcc(stream, a, d.pcap);
       launch_mcc_births(stream, a, sb.scratch, rho, gsum, d_runs, r);
     }
@@ -454,10 +471,13 @@ int Ctx::launch_collide_stage() {
       PartBufs<float> b = bufs<float>(cur);
       DsmcArgs a;
       std::memset(&a, 0, sizeof(a));
-      a.vx = b.vx + d.pbase; a.vy = b.vy + d.pbase; a.id = b.id + d.pbase;
+      a.vx = b.vx + d.pbase; a.vy = b.vy + d.pbase;
       a.offsets = sb.offsets + d.cbase; a.gmax = gmax + d.cbase;
       a.ncells = d.nx * d.ny; a.coeff = c.dsmc_coeff; a.seed = c.seed;
+      // the cell sort numbers cells tile-major; gmax and the streams are row-major
+      a.tile_shift = __builtin_ctz((unsigned)d.tile); a.tiles_y = d.tiles_y; a.ny = d.ny;
       a.st = d_st + r; a.ncoll = d_coll_counts + 4 * r;
+      a.diag = diag + d.dbase * 8;
       launch_dsmc(stream, a);
     }
   }

@@ -14,7 +14,8 @@ int fail(int code, const std::string& msg);
   do {                                                                                  \
     hipError_t e_ = (call);                                                             \
     if (e_ != hipSuccess)                                                               \
-      return ::pic::fail(PIC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return ::pic::fail(e_ == hipErrorOutOfMemory ? PIC_ERR_OOM : PIC_ERR_HIP,          \
+                         std::string(#call) + ": " + hipGetErrorString(e_));            \
   } while (0)
 
 template <typename T>

@@ -6,7 +6,8 @@
 // 3e-20 ln(E/15.8)/E above 15.8 eV), isotropic scattering in the velocity plane, births
 // appended after a deterministic ordering by parent id.
 // DSMC occupies the slot of thesisCode/dsmcCollisions/cellTrackerAdvanced/src/pic.cu:791-864
-// with NTC hard-sphere like-particle collisions on the cell-sorted arrays.
+// with NTC hard-sphere like-particle collisions on the cell-sorted arrays, candidates in
+// rounds of disjoint pairs.
 // Random streams: counter {particle id | cell, step, draw, tag}, key = seed.
 #include "launch.h"
 
@@ -83,6 +84,11 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int n_upper) {
   __syncthreads();
   if (threadIdx.x < 4 && cnt[threadIdx.x] && a.counts)
     atomicAdd((unsigned long long*)&a.counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+  // collisions of the step (elastic + excitation + ionisation) into its diagnostics row;
+  // integer-valued doubles, so the sum is exact and order-free
+  const int real = cnt[0] + cnt[1] + cnt[2];
+  if (threadIdx.x == 0 && a.diag && real)
+    atomicAdd(&a.diag[(step % PIC_DIAG_CAP) * 8 + 6], (double)real);
 }
 
 // Births of the step: order by parent id (bitonic, one workgroup), assign ids
@@ -152,127 +158,136 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   }
 }
 
-#define DSMC_WAVES 4        // waves per workgroup; each takes one cell at a time
-#define DSMC_LDS_MAX 1024    // particles of a cell held in LDS (larger cells stay in HBM)
+#define DSMC_WAVES 4  // waves per workgroup; each takes one cell at a time
 
-__device__ __forceinline__ double readlane_f64(double x, int t) {
-  const long long v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), t);
-  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), t);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+// Pair selection of a DSMC round: slot t of [0, n) through a 12-round balanced Feistel
+// network on 2*half bits (the smallest even width with 2^(2 half) >= n; round i keyed by
+// key[i % 4] ^ i*0x632BE5AB), repeated until the value is below n (cycle walking: the walk
+// stays on the cycle through t, which holds t itself, so it ends).  The pairs
+// (pi(2m), pi(2m+1)) of a round are disjoint.
+__device__ __forceinline__ uint32_t feistel_f(uint32_t r, uint32_t k, int half) {
+  uint32_t h = (r ^ k) * 0x9E3779B1u;
+  h ^= h >> 15;
+  h *= 0x85EBCA77u;
+  h ^= h >> 13;
+  return h >> (32 - half);
 }
 
-// candidate k of cell c: the pair (i, j), i != j, the acceptance uniform and the direction
-// of the post-collision relative velocity
-__device__ __forceinline__ void dsmc_candidate(const DsmcArgs& a, int c, uint32_t s32,
-                                               long long k, int n, int& i, int& j, double& u2,
-                                               double& sn, double& cs) {
-  double u[4];
-  draw4((uint32_t)c, s32, (uint32_t)k, PIC_TAG_DSMC, a.seed, u);
-  i = (int)(u[0] * n);
-  i = i > n - 1 ? n - 1 : i;
-  j = (int)(u[1] * (n - 1));
-  j = j > n - 2 ? n - 2 : j;
-  if (j >= i) ++j;
-  u2 = u[2];
-  sincos2pi_own(u[3], sn, cs);
+__device__ __forceinline__ uint32_t pair_perm(uint32_t t, uint32_t n, int half,
+                                              const uint32_t key[4]) {
+  const uint32_t mask = (1u << half) - 1u;
+  uint32_t x = t;
+  do {
+    uint32_t l = x >> half, r = x & mask;
+#pragma unroll
+    for (uint32_t i = 0; i < 12; ++i) {
+      const uint32_t f = feistel_f(r, key[i & 3] ^ (i * 0x632BE5ABu), half);
+      const uint32_t nl = r;
+      r = l ^ f;
+      l = nl;
+    }
+    x = (l << half) | r;
+  } while (x >= n);
+  return x;
 }
 
-// NTC pass of one cell: candidates in order k = 0 .. M-1; a candidate with relative speed
-// g raises the running bound gm to g, and is accepted when u2 gm < g: the pair's centre-
-// of-mass velocity is kept and its relative velocity turned to the drawn direction
-__device__ __forceinline__ void dsmc_pair(double vix, double viy, double vjx, double vjy,
-                                          double& gm, double u2, double sn, double cs,
-                                          bool& hit, float2& ni, float2& nj) {
-  const double gx = vix - vjx, gy = viy - vjy;
-  const double g = sqrt(gx * gx + gy * gy);
-  if (g > gm) gm = g;
-  hit = u2 * gm < g;
-  const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
-  const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
-  ni = make_float2((float)(cx + hx), (float)(cy + hy));
-  nj = make_float2((float)(cx - hx), (float)(cy - hy));
+__device__ __forceinline__ int pair_half_bits(uint32_t n) {
+  int b = 32 - __builtin_clz(n - 1u);  // n >= 2
+  b += b & 1;
+  return b >> 1;
 }
 
-// One wave per cell.  The cell's velocities are staged in LDS; the wave draws the random
-// numbers of 64 candidates at once (one per lane), then steps through them in order with
-// wave-uniform values (the pass is sequential: a particle may take part in several pairs),
-// so the serial chain per candidate is LDS reads and arithmetic instead of dependent HBM
-// round trips.  The candidates, their order and every operation are those of the one-cell
-// sequential pass, so the result is bit-identical to it.
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Row-major cell cx*ny + cy of cell c of the offsets' numbering: the engine's sorts number
+// cells tile-major (tile t = (cx >> sh)*tiles_y + (cy >> sh), then cx, cy inside the tile);
+// the operator's offsets are row-major already (tile_shift < 0).  The per-cell maximum
+// relative speed and the random streams are keyed by the row-major cell, so a run gives
+// the same collisions under any tiling.
+__device__ __forceinline__ int dsmc_cell_rm(int c, const DsmcArgs& a) {
+  if (a.tile_shift < 0) return c;
+  const int sh = a.tile_shift, sh2 = 2 * sh;
+  const int t = c >> sh2, w = c & ((1 << sh2) - 1);
+  const int tx = t / a.tiles_y, ty = t - tx * a.tiles_y;
+  const int cx = (tx << sh) + (w >> sh), cy = (ty << sh) + (w & ((1 << sh) - 1));
+  return cx * a.ny + cy;
+}
+
+// NTC hard-sphere collisions in rounds of disjoint pairs, one wave per cell.  The cell's
+// M candidates (M = floor(mr) + [u < frac(mr)], mr = 1/2 n (n-1) coeff gmax) are dealt to
+// rounds of P = n/2 pairs; the pairs of a round are disjoint, so its candidates go to the
+// lanes at once and read and write the velocities in HBM directly (only the candidates'
+// particles are touched).  Candidate k is accepted when u gm < g, gm = max(gmax at the
+// round's start, g); an accepted pair keeps its centre-of-mass velocity and turns its
+// relative velocity to a uniform direction.  After a round gmax = max(gmax, every g of
+// the round) (a wave maximum: exact, order-free); a further round first waits for this
+// round's stores (workgroup-scope fence: the lanes of one wave share the CU's L1).
 __global__ __launch_bounds__(64 * DSMC_WAVES) void k_dsmc(DsmcArgs a) {
-  __shared__ float2 vs[DSMC_WAVES][DSMC_LDS_MAX];
-  __shared__ unsigned int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long long step = a.st ? a.st->step - 1 : a.step;
   const uint32_t s32 = (uint32_t)step;
-  float2* sv = vs[wv];
-  unsigned int acc = 0;  // the same in every lane
+  const uint32_t sk0 = (uint32_t)a.seed, sk1 = (uint32_t)(a.seed >> 32);
+  unsigned int acc = 0;
   for (int c = blockIdx.x * DSMC_WAVES + wv; c < a.ncells; c += gridDim.x * DSMC_WAVES) {
     const int b = a.offsets[c], n = a.offsets[c + 1] - b;
     if (n < 2) continue;
-    double gm = (double)a.gmax[c];
+    const int cr = dsmc_cell_rm(c, a);
+    double gm = (double)a.gmax[cr];
     double u[4];
-    draw4((uint32_t)c, s32, 0xffffffffu, PIC_TAG_DSMC, a.seed, u);
+    draw4((uint32_t)cr, s32, 0xffffffffu, PIC_TAG_DSMC, a.seed, u);
     const double mr = 0.5 * (double)n * (double)(n - 1) * a.coeff * gm;
     const double fl = floor(mr);
-    const long long M = (long long)fl + ((u[0] < mr - fl) ? 1 : 0);
-    if (n > DSMC_LDS_MAX) {
-      // too many particles for the LDS slot: the same pass on the velocities in HBM
-      if (lane == 0) {
-        for (long long k = 0; k < M; ++k) {
-          int i, j;
-          double u2, sn, cs;
-          dsmc_candidate(a, c, s32, k, n, i, j, u2, sn, cs);
-          const long long pi = b + i, pj = b + j;
-          bool hit;
-          float2 ni, nj;
-          dsmc_pair(a.vx[pi], a.vy[pi], a.vx[pj], a.vy[pj], gm, u2, sn, cs, hit, ni, nj);
-          if (hit) {
-            a.vx[pi] = ni.x; a.vy[pi] = ni.y;
-            a.vx[pj] = nj.x; a.vy[pj] = nj.y;
-            ++acc;
-          }
-        }
-        a.gmax[c] = (float)gm;
-      }
-      continue;
-    }
-    for (int t = lane; t < n; t += 64) sv[t] = make_float2(a.vx[b + t], a.vy[b + t]);
-    for (long long k0 = 0; k0 < M; k0 += 64) {
-      int ci = 0, cj = 1;
-      double cu2 = 0.0, csn = 0.0, ccs = 1.0;
-      if (k0 + lane < M) dsmc_candidate(a, c, s32, k0 + lane, n, ci, cj, cu2, csn, ccs);
-      const int tn = (M - k0 < 64) ? (int)(M - k0) : 64;
-      for (int t = 0; t < tn; ++t) {
-        const int i = __builtin_amdgcn_readlane(ci, t), j = __builtin_amdgcn_readlane(cj, t);
-        const float2 fi = sv[i], fj = sv[j];
-        bool hit;
-        float2 ni, nj;
-        dsmc_pair(fi.x, fi.y, fj.x, fj.y, gm, readlane_f64(cu2, t), readlane_f64(csn, t),
-                  readlane_f64(ccs, t), hit, ni, nj);
-        if (hit) {
-          if (lane == 0) {
-            sv[i] = ni;
-            sv[j] = nj;
-          }
+    long long M = (long long)fl + ((u[0] < mr - fl) ? 1 : 0);
+    if (M > 0x7fffffffll) M = 0x7fffffffll;
+    if (M == 0) continue;
+    const int P = n >> 1;
+    const int half = pair_half_bits((uint32_t)n);
+    for (long long r0 = 0; r0 < M; r0 += P) {
+      uint32_t key[4] = {(uint32_t)cr, s32, 0x80000000u | (uint32_t)(r0 / P), PIC_TAG_DSMP};
+      philox4x32_10(key, sk0, sk1);
+      const int npr = (int)((M - r0 < P) ? (M - r0) : P);
+      double gr = gm;
+      for (int m = lane; m < npr; m += 64) {
+        const long long pi = b + (long long)pair_perm(2u * m, (uint32_t)n, half, key);
+        const long long pj = b + (long long)pair_perm(2u * m + 1u, (uint32_t)n, half, key);
+        double w[4];
+        draw4((uint32_t)cr, s32, (uint32_t)(r0 + m), PIC_TAG_DSMC, a.seed, w);
+        const double vix = a.vx[pi], viy = a.vy[pi], vjx = a.vx[pj], vjy = a.vy[pj];
+        const double gx = vix - vjx, gy = viy - vjy;
+        const double g = sqrt(gx * gx + gy * gy);
+        const double ge = g > gm ? g : gm;
+        if (g > gr) gr = g;
+        if (w[0] * ge < g) {
+          double sn, cs;
+          sincos2pi_own(w[1], sn, cs);
+          const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
+          const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
+          a.vx[pi] = (float)(cx + hx);
+          a.vy[pi] = (float)(cy + hy);
+          a.vx[pj] = (float)(cx - hx);
+          a.vy[pj] = (float)(cy - hy);
           ++acc;
         }
       }
+      gm = wave_max_f64(gr);
+      if (r0 + P < M) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
     }
-    for (int t = lane; t < n; t += 64) {
-      const float2 v = sv[t];
-      a.vx[b + t] = v.x;
-      a.vy[b + t] = v.y;
-    }
-    if (lane == 0) a.gmax[c] = (float)gm;
+    if (lane == 0) a.gmax[cr] = (float)gm;
   }
-  if (lane == 0 && acc) atomicAdd(&cnt, acc);
-  __syncthreads();
-  if (threadIdx.x == 0 && cnt && a.ncoll)
-    atomicAdd((unsigned long long*)a.ncoll, (unsigned long long)cnt);
+  // accepted pairs: integer sum, order-free
+  unsigned int t = acc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (lane == 0 && t && a.ncoll) atomicAdd((unsigned long long*)a.ncoll, (unsigned long long)t);
+  if (lane == 0 && t && a.diag) atomicAdd(&a.diag[(step % PIC_DIAG_CAP) * 8 + 6], (double)t);
 }
 
 }  // namespace
@@ -290,6 +305,6 @@ void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scra
 
 void launch_dsmc(hipStream_t s, const DsmcArgs& a) {
   int blocks = (a.ncells + DSMC_WAVES - 1) / DSMC_WAVES;
-  blocks = blocks < 1 ? 1 : (blocks > 16384 ? 16384 : blocks);
+  blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
   k_dsmc<<<blocks, 64 * DSMC_WAVES, 0, s>>>(a);
 }

@@ -1083,6 +1083,7 @@ __global__ __launch_bounds__(PIC_WG) void k_step_finish(const RunDesc* __restric
     row[2] = mp * s1;
     row[3] = mp * s2;
     row[5] = (double)np;
+    row[6] = 0.0;  // the collision stage of this step adds its count
     sr->step = stp + 1;
     sr->vsync = 0;
   }

@@ -213,20 +213,25 @@ struct MccArgs {
   float* birth_buf;               // [4*cap]
   int from_state_step;            // 1: take the step index from st->step
   long long step;
+  double* diag;                   // the run's diagnostics rows (may be NULL): the step's
+                                  // collisions are added to column PIC_DIAG_NCOLL
 };
 void launch_mcc(hipStream_t s, const MccArgs& a, int n_upper);
 void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scratch,
                        unsigned long long* rho, long long gsum, const RunDesc* runs, int run);
 struct DsmcArgs {
   float *vx, *vy;
-  const int* id;
-  const int* offsets;
-  float* gmax;
+  const int* offsets;             // [ncells + 1] cell ranges in the sort's numbering
+  float* gmax;                    // [ncells] per-cell maximum relative speed, row-major cell
   int ncells;
+  int tile_shift;                 // offsets in the engine's tile-major numbering with tiles of
+                                  // 1 << tile_shift cells a side; < 0: row-major already
+  int tiles_y, ny;
   double coeff;
   uint64_t seed;
   long long step;
   const RunState* st;             // may be NULL (op mode uses `step`)
   long long* ncoll;
+  double* diag;                   // the run's diagnostics rows (may be NULL)
 };
 void launch_dsmc(hipStream_t s, const DsmcArgs& a);

@@ -395,8 +395,9 @@ int pic_op_dsmc(float* vx, float* vy, const int32_t* id, const int32_t* offsets,
     return fail(PIC_ERR_ARG, "bad argument");
   DsmcArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.vx = vx; a.vy = vy; a.id = id; a.offsets = offsets; a.gmax = gmax;
-  a.ncells = ncells; a.coeff = coeff; a.seed = seed; a.step = step; a.st = nullptr;
+  (void)id;  // cells hold their particles in the caller's (canonical) order
+  a.vx = vx; a.vy = vy; a.offsets = offsets; a.gmax = gmax;
+  a.ncells = ncells; a.tile_shift = -1; a.tiles_y = 1; a.ny = 1; a.coeff = coeff; a.seed = seed; a.step = step; a.st = nullptr;
   a.ncoll = (long long*)ncoll;
   launch_dsmc((hipStream_t)stream, a);
   PIC_HIP(hipGetLastError());

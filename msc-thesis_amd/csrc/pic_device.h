@@ -151,6 +151,7 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 #define PIC_TAG_LOAD 0x4c4f4144u
 #define PIC_TAG_MCC  0x4d434331u
 #define PIC_TAG_DSMC 0x44534d43u
+#define PIC_TAG_DSMP 0x44534d50u   // DSMC round permutations
 
 __device__ __forceinline__ double u01(uint32_t r) {
   return ((double)r + 0.5) * 2.3283064365386963e-10;

@@ -27,6 +27,8 @@ PIC_COLLIDE_MCC = 1
 PIC_COLLIDE_DSMC = 2
 PIC_GRID_RHO, PIC_GRID_PHI, PIC_GRID_EX, PIC_GRID_EY = 0, 1, 2, 3
 PIC_DIAG_COLS = 8
+(PIC_DIAG_FIELD_ENERGY, PIC_DIAG_KINETIC, PIC_DIAG_PX, PIC_DIAG_PY, PIC_DIAG_MODE1,
+ PIC_DIAG_NPART, PIC_DIAG_NCOLL) = range(7)
 DIAG_NAMES = ("field_energy", "kinetic", "px", "py", "mode1", "npart", "ncoll", "reserved")
 STAGE_NAMES = ("push_deposit", "fft_rows", "fft_cols", "fft_field", "sort", "collide")
 

@@ -81,9 +81,22 @@ def mcc_256(seed=1234, ppc=128, n=256):
     return cfg, parts, 500
 
 
-def dsmc_blob(seed=1234, n=256, mean_ppc=32, coeff=0.02):
-    """Hard-sphere NTC DSMC on a Gaussian density blob (mean 32 particles/cell over the
-    box) of Maxwellian electrons."""
+def dsmc_blob(seed=1234, n=256, mean_ppc=32, coeff=3e-4, charge_scale=1e-6):
+    """Hard-sphere NTC DSMC on a Gaussian density blob (sigma = n/6 cells, mean 32
+    particles/cell over the box, 183 per cell at the centre) of Maxwellian particles
+    (unit thermal speed per component), dt = 0.2.
+
+    coeff = 3e-4 gives nu dt = (n_c - 1) coeff <g> ~ 0.1 collisions per particle per step
+    in the central cell (<g> = sqrt(pi) for unit 2D Maxwellians), the bound DSMC's
+    splitting of transport and collisions needs.  The macro-particle charge is
+    charge_scale x the plasma value -(lx ly)/N: the engine's Poisson stage neutralises
+    with a uniform background (the k = 0 mode), not with a blob of ions, so at full charge
+    the blob's space charge (field energy ~1300x its kinetic energy; the ratio scales with
+    the charge, as the field energy goes with its square and the macro-particle mass with
+    the charge itself) blows it apart within a few plasma periods.  At 1e-6 the field
+    energy is ~1.3e-3 of the kinetic energy and the blob relaxes by collisions and free
+    streaming while every stage of the step (field solve, push, cell sort, collisions)
+    still runs at full size."""
     cfg = _lib.default_config(nx=n, ny=n, lx=float(n), ly=float(n), dt=0.2, qm=-1.0,
                               precision=_lib.PIC_FP32, seed=seed, collide=_lib.PIC_COLLIDE_DSMC,
                               dsmc_coeff=coeff, dsmc_gmax0=4.0)
@@ -95,6 +108,7 @@ def dsmc_blob(seed=1234, n=256, mean_ppc=32, coeff=0.02):
     vx = rng.normal(0, 1, npart)
     vy = rng.normal(0, 1, npart)
     cfg.capacity = npart
+    cfg.qp = -charge_scale * cfg.lx * cfg.ly / npart
     return cfg, (x, y, vx, vy), 500
 
 

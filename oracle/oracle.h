@@ -151,12 +151,18 @@ int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, f
                         int32_t* id, int64_t next_id, double e_per_v2,
                         double nu_dt_per_sigma_v, double p_coll, double nu_max_dt,
                         double mass_ratio2, uint64_t seed, int64_t step, int64_t* counts);
-/* NTC hard-sphere DSMC pass over cell-sorted particles (mirror of the GPU operator): cell c
- * holds slots [offsets[c], offsets[c+1]); gmax[c] is the running maximum relative speed;
- * Philox {c, step, k, 'DSMC'} for candidate k and {c, step, 0xffffffff, 'DSMC'} for the
- * fractional candidate.  Returns the number of accepted collisions. */
+/* NTC hard-sphere DSMC pass over cell-sorted particles (mirror of the GPU operator), in
+ * rounds of disjoint pairs: cell c holds slots [offsets[c], offsets[c+1]); gmax[c] is the
+ * running maximum relative speed; candidate k is the pair (pi(2m), pi(2m+1)), m = k % (n/2),
+ * of round k / (n/2) under that round's slot permutation pi (Feistel network keyed by
+ * Philox {c, step, 0x80000000 | round, 'DSMP'}); Philox {c, step, k, 'DSMC'} gives its
+ * acceptance and direction uniforms and {c, step, 0xffffffff, 'DSMC'} the fractional
+ * candidate.  Returns the number of accepted collisions. */
 int64_t or_dsmc_step_f32(int ncells, const int32_t* offsets, float* vx, float* vy,
                          float* gmax, double coeff, uint64_t seed, int64_t step);
+/* out[t] = pi(t), t < n: the slot permutation of one DSMC round (n >= 2). */
+void or_dsmc_pair_perm(uint32_t n, int64_t cell, int64_t step, int64_t round, uint64_t seed,
+                       uint32_t* out);
 
 /* The bit-reproducible elementary functions used by the collision stages. */
 void or_sincos2pi(double u, double* s, double* c);

@@ -54,6 +54,8 @@ SIG = {
     "or_mcc_step_f32": (_I64, [_I64, _I64, _FP, _FP, _FP, _FP, _IP, _I64, _D, _D, _D, _D, _D,
                                ctypes.c_uint64, _I64, _LP]),
     "or_dsmc_step_f32": (_I64, [_I, _IP, _FP, _FP, _FP, _D, ctypes.c_uint64, _I64]),
+    "or_dsmc_pair_perm": (None, [ctypes.c_uint32, _I64, _I64, _I64, ctypes.c_uint64,
+                                 ctypes.c_void_p]),
     "or_sincos2pi": (None, [_D, _DP, _DP]),
     "or_log": (_D, [_D]),
     "or_adjoint_mt": (None, [_I, _I, _D, _D, _DP, _DP, _DP]),

@@ -77,6 +77,60 @@ def test_restart_continues_bit_for_bit(tmp_path, case):
         assert np.array_equal(g, w)
 
 
+def test_dsmc_restart_under_another_tiling(tmp_path):
+    """The DSMC per-cell maxima are stored by row-major cell: a run saved under one tiling
+    and restored under another continues exactly as the uninterrupted run."""
+    cfg, parts = dsmc_small()
+    path = tmp_path / "dsmc.ckpt"
+    with Simulation(cfg) as sim:
+        sim.set_tuning(8, 8, 0, 1)
+        sim.set_particles(*parts)
+        sim.step(9)
+        sim.sync()
+        sim.save_checkpoint(path)
+        sim.step(9)
+        sim.sync()
+        want, nwant = state(sim)
+        counts = sim.collision_counts()
+    with Simulation(cfg) as sim2:
+        sim2.set_tuning(32, 16, 0, 1)
+        sim2.load_checkpoint(path)
+        assert sim2.tuning()["tile"] == 32
+        sim2.step(9)
+        sim2.sync()
+        got, ngot = state(sim2)
+        assert sim2.collision_counts() == counts
+    assert ngot == nwant and counts[0] > 0
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_checkpoint_larger_than_the_run_is_refused(tmp_path):
+    """A checkpoint with more particles than the run's capacity is refused before the
+    run is touched."""
+    cfg, parts = landau(_lib.PIC_FP32, n=32, ppc=4)
+    path = tmp_path / "big.ckpt"
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(2)
+        sim.save_checkpoint(path)
+    small = _lib.default_config(nx=32, ny=32, lx=FOUR_PI, ly=FOUR_PI, dt=0.05,
+                                precision=_lib.PIC_FP32, seed=77, capacity=parts[0].size // 2)
+    half = [a[: parts[0].size // 2] for a in parts]
+    with Simulation(small) as sim:
+        sim.set_particles(*half)
+        sim.step(1)
+        sim.sync()
+        before = state(sim)
+        with pytest.raises(PicError) as e:
+            sim.load_checkpoint(path)
+        assert e.value.code == _lib.PIC_ERR_OVERFLOW
+        after = state(sim)
+        assert sim.current_step() == 1
+    for a, b in zip(before[0], after[0]):
+        assert np.array_equal(a, b)
+
+
 def test_mismatched_or_damaged_checkpoint_is_refused(tmp_path):
     cfg, parts = landau(_lib.PIC_FP32, n=32, ppc=4)
     path = tmp_path / "a.ckpt"

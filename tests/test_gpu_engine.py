@@ -10,6 +10,7 @@ pytestmark = pytest.mark.gpu
 
 import pic_amd  # noqa: E402
 from pic_amd import Batch, Simulation, _lib, presets  # noqa: E402
+from pic_amd._lib import lib  # noqa: E402
 
 FOUR_PI = 4.0 * math.pi
 
@@ -190,6 +191,39 @@ def test_errors_are_reported():
     assert e.value.code == _lib.PIC_ERR_OVERFLOW
     sim.check_finite()
     sim.close()
+
+
+def test_nonfinite_state_is_reported():
+    """A NaN velocity is found by the state scan (PIC_ERR_NONFINITE), with the step and
+    the other particles untouched."""
+    cfg, parts = small_landau()
+    vx = np.array(parts[2], copy=True)
+    vx[len(vx) // 3] = np.nan
+    with Simulation(cfg) as sim:
+        sim.set_particles(parts[0], parts[1], vx, parts[3])
+        with pytest.raises(pic_amd.PicError) as e:
+            sim.check_finite()
+        assert e.value.code == _lib.PIC_ERR_NONFINITE
+        sim.set_particles(*parts)
+        sim.check_finite()
+
+
+def test_out_of_device_memory_is_reported():
+    """Two FP64 runs of 2^31 - 2^16 particles each need ~430 GB, more than one MI355X
+    holds: creation fails with PIC_ERR_OOM before any buffer is allocated, and the device
+    stays usable."""
+    cfg = _lib.default_config(nx=64, ny=64, precision=_lib.PIC_FP64)
+    cfg.capacity = 0x7fff0000
+    cfgs = (_lib.pic_config * 2)(cfg, cfg)
+    ctx = ctypes.c_void_p()
+    rc = lib.pic_batch_create(cfgs, 2, ctypes.byref(ctx))
+    assert rc == _lib.PIC_ERR_OOM, lib.pic_last_error()
+    assert b"GB of device memory" in lib.pic_last_error()
+    c2, parts = small_landau()
+    with Simulation(c2) as sim:
+        sim.set_particles(*parts)
+        sim.step(2)
+        sim.sync()
 
 
 def damping_rate(diag, dt, tmax=20.0):
