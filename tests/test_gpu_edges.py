@@ -39,15 +39,12 @@ def config(nx, ny, lx, ly, n, dt=0.05):
                                seed=11, capacity=n)
 
 
-def assert_close_to_mirror(got, ref, lx, ly, atol=2e-5):
-    x, y, vx, vy = got
-    # positions compared on the circle: a particle on the edge may sit at 0 or at L - ulp
-    dx = np.abs(x - ref[0])
-    dy = np.abs(y - ref[1])
-    assert np.minimum(dx, lx - dx).max() <= atol
-    assert np.minimum(dy, ly - dy).max() <= atol
-    np.testing.assert_allclose(vx, ref[2], atol=atol)
-    np.testing.assert_allclose(vy, ref[3], atol=atol)
+def assert_equal_to_mirror(got, ref):
+    """The FP32 path reproduces the oracle's FP32 mirror bit for bit (positions, velocities):
+    same fixed-point charge, FP64 field within rounding of the oracle's (its FP32 values
+    then round alike), same operation order in the push."""
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
 
 
 def test_one_particle(oracle):
@@ -56,7 +53,7 @@ def test_one_particle(oracle):
     parts = (np.array([1.3]), np.array([7.9]), np.array([0.7]), np.array([-1.1]))
     got, diag = run_engine(cfg, parts, 25)
     ref, diag_r = run_mirror(oracle, cfg, parts, 25)
-    assert_close_to_mirror(got, ref, cfg.lx, cfg.ly)
+    assert_equal_to_mirror(got, ref)
     np.testing.assert_allclose(diag[:, 1], diag_r[:, 1], rtol=1e-5)
 
 
@@ -75,7 +72,7 @@ def test_particles_on_the_domain_edges(oracle):
     cfg = config(64, 64, L, L, x.size)
     got, _ = run_engine(cfg, (x, y, vx, vy), 12)
     ref, _ = run_mirror(oracle, cfg, (x, y, vx, vy), 12)
-    assert_close_to_mirror(got, ref, L, L)
+    assert_equal_to_mirror(got, ref)
     assert got[0].min() >= 0.0 and got[0].max() < L
     assert got[1].min() >= 0.0 and got[1].max() < L
 
@@ -95,7 +92,7 @@ def test_every_particle_in_one_cell(oracle, tuning):
     cfg = config(64, 64, FOUR_PI, FOUR_PI, n)
     got, diag = run_engine(cfg, (x, y, vx, vy), 10, tuning)
     ref, diag_r = run_mirror(oracle, cfg, (x, y, vx, vy), 10)
-    assert_close_to_mirror(got, ref, cfg.lx, cfg.ly, atol=5e-5)
+    assert_equal_to_mirror(got, ref)
     np.testing.assert_allclose(diag[:, 0], diag_r[:, 0], rtol=1e-4)
 
 
@@ -105,7 +102,7 @@ def test_non_square_grid_and_domain(oracle):
     cfg = config(nx, ny, lx, ly, parts[0].size)
     got, diag = run_engine(cfg, parts, 20)
     ref, diag_r = run_mirror(oracle, cfg, parts, 20)
-    assert_close_to_mirror(got, ref, lx, ly)
+    assert_equal_to_mirror(got, ref)
     np.testing.assert_allclose(diag[:, 0], diag_r[:, 0], rtol=1e-5)
     np.testing.assert_allclose(diag[:, 1], diag_r[:, 1], rtol=1e-5)
 
@@ -184,7 +181,7 @@ def test_fast_particles_in_the_extrapolated_sort_keys(oracle):
         for u, v in zip(base, got):
             assert np.array_equal(u, v)
     ref, _ = run_mirror(oracle, cfg, (x, y, vx, vy), 9)
-    assert_close_to_mirror(base, ref, L, L, atol=2e-3)
+    assert_equal_to_mirror(base, ref)
 
 
 def test_fp64_trajectories_do_not_depend_on_the_sort():

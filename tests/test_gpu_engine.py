@@ -56,10 +56,9 @@ def test_steps_match_fp32_mirror(oracle):
         sim.step(nsteps)
         x, y, vx, vy = sim.particles(sync=True)
         diag = sim.diagnostics(0, nsteps)
-    np.testing.assert_allclose(x, ref[0], atol=2e-5)
-    np.testing.assert_allclose(y, ref[1], atol=2e-5)
-    np.testing.assert_allclose(vx, ref[2], atol=2e-5)
-    np.testing.assert_allclose(vy, ref[3], atol=2e-5)
+    # the FP32 path reproduces the oracle's FP32 mirror bit for bit
+    for g, r in zip((x, y, vx, vy), ref):
+        assert np.array_equal(g, r)
     for col in (0, 1):  # field and kinetic energy
         np.testing.assert_allclose(diag[:, col], diag_r[:, col], rtol=1e-5)
     np.testing.assert_allclose(diag[:, 4], diag_r[:, 4], rtol=1e-4, atol=1e-9)
