@@ -269,6 +269,29 @@ int pic_op_dsmc(float* vx, float* vy, const int32_t* id, const int32_t* offsets,
                 int ncells, float* gmax, double coeff, uint64_t seed, int64_t step,
                 int64_t* ncoll, void* stream);
 
+/* The reference's own collide arithmetic (FP64 device arrays), driven by an injected
+ * uniform stream u[nu] (device) in the order the reference draws it: one std::mt19937 over
+ * the particles in index order (std::uniform_real_distribution<double>{0, 1} reproduces
+ * the reference's host stream).  A particle that draws takes r = u[k] and, when prob > r,
+ * six more uniforms: v = (2(u1 + u2 + u3 - 1.5), 2(u4 + u5 + u6 - 1.5)).  The operators
+ * work out on the device which uniforms each particle takes (a chain over the stream,
+ * resolved in parallel), write *consumed (host) = uniforms used, so the next call goes on
+ * at u + consumed, and fail with PIC_ERR_ARG when the stream is too short.  Synchronous.
+ *
+ * MCC: every particle draws.  thesisCode/mccCollisions/src/particle.cc:112-127 (prob 0.8
+ * there; the reference GPU kernel mccCollisions/src/pic.cu:435-463 uses 0.22). */
+int pic_op_mcc_reference(double* vx, double* vy, int64_t n, double prob, const double* u,
+                         int64_t nu, int64_t* consumed, void* stream);
+/* DSMC: particle i draws when the nearest other particle t of its cell ((int)(x/dx),
+ * (int)(y/dy); strictly nearer wins, the lower index on ties) has t > i; a collision sets
+ * v_i = (m1, m2) and v_t = (-m1, -m2), later particles overwriting earlier ones as in the
+ * reference's loop.  Positions must lie in [0, lx) x [0, ly).  *ncoll (host, may be NULL)
+ * = collisions.  thesisCode/dsmcCollisions/cellTrackerAdvanced/src/main.cc:269-350 (q = 0.2
+ * there). */
+int pic_op_dsmc_reference(const double* x, const double* y, double* vx, double* vy,
+                          int64_t n, const pic_grid* g, double prob, const double* u,
+                          int64_t nu, int64_t* consumed, int64_t* ncoll, void* stream);
+
 /* ----------------------------------------------------------------- checkpoint / restart
  * No reference counterpart (its -f flag only truncates a file, CLd/src/main.cc:101-108).
  * A versioned binary snapshot of one run: its config, step, velocity phase, id counter,

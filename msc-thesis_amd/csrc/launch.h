@@ -235,3 +235,27 @@ struct DsmcArgs {
   double* diag;                   // the run's diagnostics rows (may be NULL)
 };
 void launch_dsmc(hipStream_t s, const DsmcArgs& a);
+
+// ---- the reference's own collide arithmetic on an injected uniform stream (k_refcollide.hip)
+#define REF_CHAIN_CHUNK 1024  // stream positions per chunk of the consumer-start resolution
+struct RefChainBufs {
+  int2* maps;            // [7 * nchunks] (exit offset, consumers) per chunk and entry offset
+  int* entry;            // [nchunks]
+  long long* base;       // [nchunks] first consumer starting in the chunk
+  long long* total;      // [1] consumers the stream starts
+  long long* start;      // [m] stream position of each consumer
+  long long* consumed;   // [1] uniforms taken by the m consumers, -1 if the stream is short
+};
+void launch_ref_chain(hipStream_t s, const double* u, long long nu, double prob, long long m,
+                      RefChainBufs b);
+void launch_mcc_reference(hipStream_t s, double* vx, double* vy, long long n, double prob,
+                          const double* u, const long long* start);
+void launch_ref_cell_keys(hipStream_t s, const double* x, const double* y, long long n, int nx,
+                          int ny, double dx, double dy, int* keys, int* bad);
+void launch_ref_nearest(hipStream_t s, const double* x, const double* y, long long n,
+                        const int* keys, const int* perm, const int* offsets, int* target,
+                        int* drawkey);
+void launch_dsmc_reference(hipStream_t s, double* vx, double* vy, long long n, long long m,
+                           double prob, const double* u, const long long* start,
+                           const int* consumers, const int* target, double* m1, double* m2,
+                           int* self, int* lastw, unsigned long long* ncoll);

@@ -123,6 +123,9 @@ SIGNATURES = {
     "pic_op_sort": (_I, [_P, _I64, _I, _P, _P, _P, _P]),
     "pic_op_deposit": (_I, [_P, _P, _I64, _I, _GRID, _D, _P, _P]),
     "pic_op_deposit_noalloc": (_I, [_P, _P, _I64, _I, _GRID, _D, _P, _P]),
+    "pic_op_mcc_reference": (_I, [_P, _P, _I64, _D, _P, _I64, ctypes.POINTER(_I64), _P]),
+    "pic_op_dsmc_reference": (_I, [_P, _P, _P, _P, _I64, _GRID, _D, _P, _I64,
+                                   ctypes.POINTER(_I64), ctypes.POINTER(_I64), _P]),
     "pic_op_poisson": (_I, [_P, _P, _GRID, _P]),
     "pic_op_efield": (_I, [_P, _P, _P, _GRID, _P]),
     "pic_op_gather_push": (_I, [_P, _P, _P, _P, _I64, _I, _P, _P, _GRID, _D, _D, _D, _P, _P]),

@@ -1024,6 +1024,90 @@ void or_dsmc_pair_perm(uint32_t n, int64_t cell, int64_t step, int64_t round, ui
   for (uint32_t t = 0; t < n; ++t) out[t] = pair_perm(t, n, half, key);
 }
 
+// ------------------------------------------------- the reference's own collide arithmetic
+// Uniform stream of the reference's host collisions: std::uniform_real_distribution<double>
+// {0, 1} over std::mt19937 gen{seed} (thesisCode/mccCollisions/src/main.cc:136,
+// particle.cc:117; thesisCode/dsmcCollisions/cellTrackerAdvanced/src/main.cc:327-328).
+void or_std_uniform_stream(uint32_t seed, int64_t count, double* out) {
+  std::mt19937 gen{seed};
+  std::uniform_real_distribution<double> dist{0.0, 1.0};
+  for (int64_t k = 0; k < count; ++k) out[k] = dist(gen);
+}
+
+// MCC collide of thesisCode/mccCollisions/src/particle.cc:112-127 called for particles
+// 0..n-1 in order (main.cc:181-183) on the uniform stream u: r = u[k]; when prob > r the
+// velocity becomes (2(u1 + u2 + u3 - 1.5), 2(u4 + u5 + u6 - 1.5)) with the next six
+// uniforms.  Returns the uniforms consumed, or -1 when the stream ends first.
+int64_t or_mcc_reference(int64_t n, double* vx, double* vy, double prob, const double* u,
+                         int64_t nu) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (k >= nu) return -1;
+    const double r = u[k];
+    if (prob > r) {
+      if (k + 7 > nu) return -1;
+      vx[i] = 2.0 * (u[k + 1] + u[k + 2] + u[k + 3] - 1.5);
+      vy[i] = 2.0 * (u[k + 4] + u[k + 5] + u[k + 6] - 1.5);
+      k += 7;
+    } else {
+      k += 1;
+    }
+  }
+  return k;
+}
+
+// DSMC collide of thesisCode/dsmcCollisions/cellTrackerAdvanced/src/main.cc:288-350 on
+// cell lists built as cellTracking does (:269-278, ascending particle index; the cell key
+// here is cx*ny + cy, the reference GPU's, which partitions the particles like the host's
+// cx + cy*ny when nx = ny): for each particle i in order, the nearest other particle t of
+// its cell (Euclidean distance, strictly smaller wins, first found on ties, initial bound
+// 1e8); kept only when t > i; then r = u[k] and, when prob > r,
+// v_i = (m1, m2), v_t = (-m1, -m2) with m = 2(u1 + u2 + u3 - 1.5) from the next six
+// uniforms.  cx = (int)(x/dx) as the reference Particle keeps it (particle.cc:23-24).
+// Returns the uniforms consumed (-1 when the stream ends first); *ncoll = collisions.
+int64_t or_dsmc_reference(int64_t n, const double* x, const double* y, double* vx,
+                          double* vy, int nx, int ny, double lx, double ly, double prob,
+                          const double* u, int64_t nu, int64_t* ncoll) {
+  const double dx = lx / nx, dy = ly / ny;
+  std::vector<std::vector<int64_t>> cells((size_t)nx * ny);
+  for (int64_t i = 0; i < n; ++i) {
+    const int cx = (int)(x[i] / dx), cy = (int)(y[i] / dy);
+    cells[(size_t)cx * ny + cy].push_back(i);
+  }
+  int64_t k = 0, nc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int cx = (int)(x[i] / dx), cy = (int)(y[i] / dy);
+    double nearest = 1e8;
+    int64_t target = -1;
+    for (int64_t t : cells[(size_t)cx * ny + cy]) {
+      if (t == i) continue;
+      const double d = std::sqrt((x[i] - x[t]) * (x[i] - x[t]) + (y[i] - y[t]) * (y[i] - y[t]));
+      if (d < nearest) {
+        nearest = d;
+        target = t;
+      }
+    }
+    if (target < i) continue;
+    if (k >= nu) return -1;
+    const double r = u[k];
+    if (prob > r) {
+      if (k + 7 > nu) return -1;
+      const double m1 = 2.0 * (u[k + 1] + u[k + 2] + u[k + 3] - 1.5);
+      const double m2 = 2.0 * (u[k + 4] + u[k + 5] + u[k + 6] - 1.5);
+      vx[i] = m1;
+      vy[i] = m2;
+      vx[target] = -m1;
+      vy[target] = -m2;
+      k += 7;
+      ++nc;
+    } else {
+      k += 1;
+    }
+  }
+  if (ncoll) *ncoll = nc;
+  return k;
+}
+
 // ---------------------------------------------------------------- adjoint
 // Reverse mode of the continuous-weight FP64 run, with the whole trajectory kept in memory
 // (small cases).  E = M rho with M = -Dc P (P: spectral Green operator, symmetric; Dc:

@@ -164,6 +164,19 @@ int64_t or_dsmc_step_f32(int ncells, const int32_t* offsets, float* vx, float* v
 void or_dsmc_pair_perm(uint32_t n, int64_t cell, int64_t step, int64_t round, uint64_t seed,
                        uint32_t* out);
 
+/* The reference's own collide arithmetic on an injected uniform stream u[nu] (KEEP list):
+ *   or_std_uniform_stream: std::uniform_real_distribution<double>{0,1} over std::mt19937{seed};
+ *   or_mcc_reference: thesisCode/mccCollisions/src/particle.cc:112-127 for particles 0..n-1;
+ *   or_dsmc_reference: thesisCode/dsmcCollisions/cellTrackerAdvanced/src/main.cc:288-350
+ *     (nearest partner in the cell, t > i, +-(Bird normal) pair assignment).
+ * Both return the uniforms consumed, -1 when the stream is too short. */
+void or_std_uniform_stream(uint32_t seed, int64_t count, double* out);
+int64_t or_mcc_reference(int64_t n, double* vx, double* vy, double prob, const double* u,
+                         int64_t nu);
+int64_t or_dsmc_reference(int64_t n, const double* x, const double* y, double* vx,
+                          double* vy, int nx, int ny, double lx, double ly, double prob,
+                          const double* u, int64_t nu, int64_t* ncoll);
+
 /* The bit-reproducible elementary functions used by the collision stages. */
 void or_sincos2pi(double u, double* s, double* c);
 double or_log(double x);

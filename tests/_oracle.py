@@ -56,6 +56,9 @@ SIG = {
     "or_dsmc_step_f32": (_I64, [_I, _IP, _FP, _FP, _FP, _D, ctypes.c_uint64, _I64]),
     "or_dsmc_pair_perm": (None, [ctypes.c_uint32, _I64, _I64, _I64, ctypes.c_uint64,
                                  ctypes.c_void_p]),
+    "or_std_uniform_stream": (None, [ctypes.c_uint32, _I64, _DP]),
+    "or_mcc_reference": (_I64, [_I64, _DP, _DP, _D, _DP, _I64]),
+    "or_dsmc_reference": (_I64, [_I64, _DP, _DP, _DP, _DP, _I, _I, _D, _D, _D, _DP, _I64, _LP]),
     "or_sincos2pi": (None, [_D, _DP, _DP]),
     "or_log": (_D, [_D]),
     "or_adjoint_mt": (None, [_I, _I, _D, _D, _DP, _DP, _DP]),
@@ -238,3 +241,21 @@ def load():
         fn.restype = res
         fn.argtypes = args
     return Oracle(lib)
+
+
+REF_MCC_SO = os.path.join(ORACLE_DIR, "_ref", "libref_mcc.so")
+REF_DSMC_SO = os.path.join(ORACLE_DIR, "_ref", "libref_dsmc.so")
+
+
+def load_ref_collide():
+    """(mcc, dsmc) ctypes handles of the reference's own collide code, or None if not built."""
+    if not (os.path.exists(REF_MCC_SO) and os.path.exists(REF_DSMC_SO)):
+        return None
+    mcc = ctypes.CDLL(REF_MCC_SO)
+    mcc.ref_mcc_collide.argtypes = [_I64, _DP, _DP, ctypes.c_uint32, _I]
+    mcc.ref_mcc_collide.restype = None
+    dsmc = ctypes.CDLL(REF_DSMC_SO)
+    dsmc.ref_dsmc_collide.argtypes = [_I64, _DP, _DP, _DP, _DP, _I, _I, _D, _D,
+                                      ctypes.c_uint32, _I]
+    dsmc.ref_dsmc_collide.restype = None
+    return mcc, dsmc
