@@ -9,7 +9,9 @@ Prints ONE JSON line.  Extra keys: roofline (dominant kernel, HIP-event timed), 
 (the CPU oracle on a bounded sample, one thread) and cpu_baseline_all_cores (the same on
 every host thread the process may use), stages (ms per step per stage), batched_vs_sequential
 (48-run ragged ensemble), adjoint_over_forward (128^2 x 32 ppc, 100 steps) and mcc_256
-(the collision workload: 256^2, 8.4M electrons, null-collision MCC).
+(the collision workload: 256^2, 8.4M electrons, null-collision MCC), dsmc_256 (the
+DSMC blob: 256^2, 2.1M particles) and two_stream_64 (config 1: 64^2, 409,600 FP64
+particles), each with the oracle's CPU rate on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -41,7 +43,8 @@ def parse():
                     help="run only the streaming reference: an in-place read-modify-write of "
                          "four 64 MiB FP32 arrays (the plain push's particle state) with the "
                          "caches flushed before each pass")
-    ap.add_argument("--only", choices=["batch", "adjoint", "mcc"], default=None,
+    ap.add_argument("--only", choices=["batch", "adjoint", "mcc", "dsmc", "two_stream"],
+                    default=None,
                     help="run only the ensemble comparison or only the adjoint timing "
                          "(for profiling those legs)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -165,6 +168,143 @@ def mcc_rate(steps=100, warmup=10):
             "grid": [cfg.nx, cfg.ny]}
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import load
+    return load()
+
+
+def two_stream_rate(warmup=10):
+    """Config 1: cold two-stream, 64 x 64, 409,600 FP64 particles, FP64 everything, dt = 0.1,
+    the configured 200 steps timed after warm-up steps; the growth rate of |rho_hat(k)| fitted
+    over the linear phase t in [8, 16] of the timed run (cold beams, k v0 = 0.5:
+    gamma = 0.3406).  CPU baseline: the oracle's FP64 step (or_run_f64, one thread) over the
+    whole 200-step run on the same particles."""
+    import numpy as np
+    import torch
+    from pic_amd import Simulation, presets
+    cfg, parts, nsteps = presets.two_stream_64()
+    n = parts[0].size
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        sim.step(warmup)
+        sim.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(nsteps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+    with Simulation(cfg) as sim:  # the growth fit needs the run from t = 0
+        sim.set_particles(*parts)
+        sim.step(nsteps)
+        diag = sim.diagnostics(0, nsteps)
+    t = np.arange(nsteps) * cfg.dt
+    m = (t >= 8) & (t <= 16)
+    gamma = float(np.polyfit(t[m], np.log(diag[m, 4]), 1)[0])
+    out = {"particle_steps_per_s": n * nsteps / dt, "ms_per_step": dt * 1e3 / nsteps,
+           "particles": n, "steps": nsteps, "grid": [cfg.nx, cfg.ny], "dtype": "f64",
+           "growth_rate": gamma, "growth_rate_theory": 0.3406}
+    return out
+
+
+def two_stream_cpu(nsteps=200):
+    from pic_amd import presets
+    cfg, parts, _ = presets.two_stream_64()
+    o = _oracle()
+    arrs = [a.copy() for a in parts]
+    n = arrs[0].size
+    t0 = time.perf_counter()
+    o.run_f64(*arrs, cfg.nx, cfg.ny, cfg.lx, cfg.ly, cfg.dt, cfg.qm, -(cfg.lx * cfg.ly) / n,
+              nsteps)
+    dt = time.perf_counter() - t0
+    return {"value": n * nsteps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"the whole {nsteps}-step 64x64 x {n} particle two-stream run through "
+                      "the FP64 oracle (or_run_f64, 1 thread)", "seconds": dt}
+
+
+def _collide_cpu(cfg, parts, nsteps):
+    """The oracle composition of the engine's step with collisions (deposit, FP64 Poisson,
+    E, FP32 push, then MCC per particle or DSMC per cell on the cell-sorted particles), one
+    thread, on the workload's own particles for nsteps steps."""
+    import ctypes
+    import numpy as np
+    from pic_amd import _lib
+    o = _oracle()
+    n = parts[0].size
+    cap = max(int(cfg.capacity), n)
+    x, y, vx, vy = (np.zeros(cap, np.float32) for _ in range(4))
+    for a, b in zip((x, y, vx, vy), parts):
+        a[:n] = b
+    ids = np.zeros(cap, np.int32)
+    ids[:n] = np.arange(n)
+    qp = cfg.qp if cfg.qp != 0 else -(cfg.lx * cfg.ly) / n
+    G = cfg.nx * cfg.ny
+    gmax = np.full(G, cfg.dsmc_gmax0, np.float32)
+    counts = np.zeros(4, np.int64)
+    if cfg.collide == _lib.PIC_COLLIDE_MCC:
+        kE, kMe = 1.602176634e-19, 9.1093837015e-31
+        numax = _lib.lib.pic_mcc_nu_max(ctypes.byref(cfg))
+        mp = (0.5 * kMe * cfg.mcc_v_unit ** 2 / kE,
+              cfg.mcc_ng * cfg.mcc_v_unit * cfg.mcc_dt_phys * cfg.dt,
+              1.0 - math.exp(-numax), numax, 2.0 * kMe / cfg.mcc_neutral_mass)
+    m = n
+    t0 = time.perf_counter()
+    for s in range(nsteps):
+        xs, ys, vxs, vys = x[:m], y[:m], vx[:m], vy[:m]
+        acc = o.deposit_fixed(xs, ys, cfg.nx, cfg.ny, cfg.lx, cfg.ly)
+        rho = o.fixed_to_rho(acc, cfg.nx, cfg.ny, cfg.lx, cfg.ly, qp)
+        ex, ey = o.efield(o.poisson(rho, cfg.lx, cfg.ly), cfg.lx, cfg.ly)
+        o.push_f32(xs, ys, vxs, vys, ex, ey, cfg.lx, cfg.ly, cfg.dt, cfg.qm,
+                   0.5 if s == 0 else 1.0)
+        if cfg.collide == _lib.PIC_COLLIDE_DSMC:
+            keys = o.cell_keys(xs, ys, cfg.nx, cfg.ny, cfg.lx, cfg.ly)
+            perm, _, off = o.stable_sort(keys, G)
+            svx, svy = np.ascontiguousarray(vxs[perm]), np.ascontiguousarray(vys[perm])
+            counts[0] += o.or_dsmc_step_f32(G, o.p(off), o.p(svx), o.p(svy), o.p(gmax),
+                                            cfg.dsmc_coeff, cfg.seed, s)
+            vx[perm], vy[perm] = svx, svy
+        else:
+            m = o.or_mcc_step_f32(m, cap, o.p(x), o.p(y), o.p(vx), o.p(vy), o.p(ids), m,
+                                  *mp, cfg.seed, s, o.p(counts))
+    dt = time.perf_counter() - t0
+    return n * nsteps / dt, dt
+
+
+def dsmc_rate(steps=100, warmup=10):
+    """The DSMC workload: 256 x 256, a Gaussian blob of 2,097,152 particles (183 per cell at
+    the centre), NTC hard spheres per cell on the cell-sorted state every step (nu dt ~ 0.1
+    at the centre); particle-steps/s of the whole step after warm-up steps."""
+    import torch
+    from pic_amd import Simulation, presets
+    cfg, parts, _ = presets.dsmc_blob()
+    with Simulation(cfg) as sim:
+        sim.set_particles(*parts)
+        n = sim.num_particles()
+        sim.step(warmup)
+        sim.sync()
+        torch.cuda.synchronize()
+        c0 = sim.collision_counts()[0]
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        c1 = sim.collision_counts()[0]
+    return {"particle_steps_per_s": n * steps / dt, "ms_per_step": dt * 1e3 / steps,
+            "particles": n, "steps": steps, "grid": [cfg.nx, cfg.ny],
+            "collisions_per_step": (c1 - c0) / steps}
+
+
+def collide_cpu(which, nsteps):
+    from pic_amd import presets
+    cfg, parts, _ = presets.mcc_256() if which == "mcc" else presets.dsmc_blob()
+    rate, dt = _collide_cpu(cfg, parts, nsteps)
+    what = ("null-collision MCC on the 8,388,608-electron 256x256 workload" if which == "mcc"
+            else "NTC DSMC on the 2,097,152-particle 256x256 blob")
+    return {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{nsteps} steps of the {what} through the oracle's stages (FP32 mirror "
+                      "push, FP64 Poisson, the collision step), 1 thread", "seconds": dt}
+
+
 def batch_ratio(steps=300):
     """48 ragged runs: one batched context vs the same runs one at a time.  Both sides are
     autotuned first (the batch over tile/halo/sort cadence for all runs at once, each
@@ -233,6 +373,12 @@ def main():
         return
     if args.only == "mcc":
         print(json.dumps({"mcc_256": mcc_rate()}))
+        return
+    if args.only == "dsmc":
+        print(json.dumps({"dsmc_256": dsmc_rate()}))
+        return
+    if args.only == "two_stream":
+        print(json.dumps({"two_stream_64": two_stream_rate()}))
         return
     if args.only == "adjoint":
         from pic_amd import _lib
@@ -324,8 +470,22 @@ def main():
     if not args.no_mcc:
         try:
             line["mcc_256"] = mcc_rate()
+            if not args.no_cpu:
+                line["mcc_256"]["cpu_baseline"] = collide_cpu("mcc", 3)
         except Exception as e:  # reported, not hidden
             line["mcc_256"] = {"error": repr(e)}
+        try:
+            line["dsmc_256"] = dsmc_rate()
+            if not args.no_cpu:
+                line["dsmc_256"]["cpu_baseline"] = collide_cpu("dsmc", 3)
+        except Exception as e:
+            line["dsmc_256"] = {"error": repr(e)}
+    try:
+        line["two_stream_64"] = two_stream_rate()
+        if not args.no_cpu:
+            line["two_stream_64"]["cpu_baseline"] = two_stream_cpu()
+    except Exception as e:
+        line["two_stream_64"] = {"error": repr(e)}
     print(json.dumps(line))
 
 

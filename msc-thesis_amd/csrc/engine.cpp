@@ -159,7 +159,7 @@ int Ctx::build_items() {
     d.cell_item_begin = (int)citems.size();
     for (int c = 0; c < d.nx * d.ny; c += PIC_CELL_CHUNK) citems.push_back(make_int2(r, c));
     d.cell_item_end = (int)citems.size();
-    for (int c = 0; c < d.nx * d.ny; c += 64) segitems.push_back(make_int2(r, c));
+    for (int c = 0; c < d.nx * d.ny; c += PIC_SEG_CELLS) segitems.push_back(make_int2(r, c));
     d.push_item_begin = (int)push_items.size();
     for (int tx = 0; tx < d.tiles_x; ++tx)
       for (int ty = 0; ty < d.tiles_y; ++ty)

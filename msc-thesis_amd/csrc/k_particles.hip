@@ -65,10 +65,12 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict_
                                                       const int2* __restrict__ items) {
   __shared__ int hist[HWIN];
   __shared__ int red[8];
-  const int2 it = items[blockIdx.x];
+  // slice blockIdx % PIC_SORT_SLICES of chunk item blockIdx / PIC_SORT_SLICES
+  const int2 it = items[blockIdx.x / PIC_SORT_SLICES];
   const RunDesc rd = runs[it.x];
   const int np = st[it.x].np;
-  const int p0 = it.y, p1 = min(p0 + PIC_CHUNK, np);
+  const int p0 = it.y + (blockIdx.x % PIC_SORT_SLICES) * (PIC_CHUNK / PIC_SORT_SLICES);
+  const int p1 = min(p0 + PIC_CHUNK / PIC_SORT_SLICES, np);
   if (p0 >= p1) return;  // uniform across the workgroup
   const long long pb = rd.pbase;
   const int nx = rd.nx, ny = rd.ny;
@@ -193,11 +195,12 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_scatter(const RunDesc* __restri
                                                          const int* __restrict__ offsets,
                                                          int* __restrict__ perm,
                                                          const int2* __restrict__ items) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items[blockIdx.x / PIC_SORT_SLICES];
   const RunDesc rd = runs[it.x];
   const int np = st[it.x].np;
-  const int p1 = min(it.y + PIC_CHUNK, np);
-  for (int p = it.y + threadIdx.x; p < p1; p += PIC_WG) {
+  const int p0 = it.y + (blockIdx.x % PIC_SORT_SLICES) * (PIC_CHUNK / PIC_SORT_SLICES);
+  const int p1 = min(p0 + PIC_CHUNK / PIC_SORT_SLICES, np);
+  for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) {
     int d = offsets[rd.cbase + keys[rd.pbase + p]] + rank[rd.pbase + p];
     perm[rd.pbase + d] = p;
   }
@@ -225,8 +228,8 @@ __device__ __forceinline__ unsigned long long seg_key(const int* id, long long p
 // Canonical order inside each cell: ascending key (id, or the source index when id is
 // NULL, which makes the sort stable).  One wavefront per cell: <= 64 entries by a bitonic
 // network across lanes, <= 256 entries by counting ranks (4 entries per lane, keys are
-// unique); larger cells are queued for k_segsort_big.  Items cover SEG_CELLS cells.
-#define SEG_CELLS 64
+// unique); larger cells are queued for k_segsort_big.  Items cover PIC_SEG_CELLS cells
+// (one per wave).
 __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restrict__ runs,
                                                           const int* __restrict__ offsets,
                                                           int* __restrict__ perm,
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restr
   const RunDesc rd = runs[it.x];
   const int G = rd.nx * rd.ny;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cend = min(it.y + SEG_CELLS, G);
+  const int cend = min(it.y + PIC_SEG_CELLS, G);
   const long long pb = rd.pbase;
   for (int c = it.y + w; c < cend; c += 4) {
     const int b = offsets[rd.cbase + c], e = offsets[rd.cbase + c + 1];
@@ -334,12 +337,13 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict
                                                        const int* __restrict__ perm,
                                                        PartBufs<R> src, PartBufs<R> dst,
                                                        const int2* __restrict__ items) {
-  const int2 it = items[blockIdx.x];
+  const int2 it = items[blockIdx.x / PIC_SORT_SLICES];
   const RunDesc rd = runs[it.x];
   const int np = st[it.x].np;
-  const int p1 = min(it.y + PIC_CHUNK, np);
+  const int d0 = it.y + (blockIdx.x % PIC_SORT_SLICES) * (PIC_CHUNK / PIC_SORT_SLICES);
+  const int p1 = min(d0 + PIC_CHUNK / PIC_SORT_SLICES, np);
   const long long pb = rd.pbase;
-  for (int d = it.y + threadIdx.x; d < p1; d += PIC_WG) {
+  for (int d = d0 + threadIdx.x; d < p1; d += PIC_WG) {
     const long long s = pb + perm[pb + d];
     dst.x[pb + d] = src.x[s];
     dst.y[pb + d] = src.y[s];
@@ -990,14 +994,16 @@ void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, Pa
                  PartBufs<R> dst, SortBufs sb, const int2* pitems, int npitems,
                  const int2* citems, int ncitems, long long ctotal, bool move_data) {
   launch_fill_i32(s, sb.counts, ctotal, 0);
-  k_sort_rank<R, true><<<npitems, PIC_WG, 0, s>>>(runs, st, src.x, src.y, nullptr, sb.keys,
+  k_sort_rank<R, true><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, src.x, src.y,
+                                                                   nullptr, sb.keys,
                                                  sb.rank, sb.counts, pitems);
   sort_scan(s, runs, st, nruns, sb, citems, ncitems);
-  k_sort_scatter<<<npitems, PIC_WG, 0, s>>>(runs, st, sb.keys, sb.rank, sb.offsets, sb.perm,
-                                            pitems);
+  k_sort_scatter<<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, sb.keys, sb.rank,
+                                                              sb.offsets, sb.perm, pitems);
   sort_tail(s, runs, st, src.id, sb, citems, ncitems);
   if (move_data)
-    k_sort_apply<R><<<npitems, PIC_WG, 0, s>>>(runs, st, sb.perm, src, dst, pitems);
+    k_sort_apply<R><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, sb.perm, src, dst,
+                                                                 pitems);
   k_toff_from_cells<<<64, PIC_WG, 0, s>>>(runs, sb.offsets, sb.toff, nruns);
 }
 
@@ -1029,10 +1035,12 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
                            SortBufs sb, const int2* pitems, int npitems, const int2* citems,
                            int ncitems, long long ctotal) {
   launch_fill_i32(s, sb.counts, ctotal, 0);
-  k_sort_rank<float, false><<<npitems, PIC_WG, 0, s>>>(runs, st, nullptr, nullptr, keys,
+  k_sort_rank<float, false><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, nullptr,
+                                                                        nullptr, keys,
                                                       sb.keys, sb.rank, sb.counts, pitems);
   sort_scan(s, runs, st, 1, sb, citems, ncitems);
-  k_sort_scatter<<<npitems, PIC_WG, 0, s>>>(runs, st, keys, sb.rank, sb.offsets, sb.perm, pitems);
+  k_sort_scatter<<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, keys, sb.rank, sb.offsets,
+                                                              sb.perm, pitems);
   sort_tail(s, runs, st, nullptr, sb, citems, ncitems);
 }
 

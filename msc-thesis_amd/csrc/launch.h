@@ -9,6 +9,12 @@
 //   tile items   int4 {run, tx, ty, split}; tx < 0 marks a tail chunk (start in .z)
 //   row items    int2 {run, row}
 #define PIC_CELL_CHUNK 2048
+// cell sort: cells per item of the within-cell sort (one wave per cell, four waves per
+// workgroup), and the slices of a particle chunk item that the rank, scatter and apply
+// kernels give one workgroup each (more workgroups than chunks: the blob-like runs of the
+// collision stages need the latency hiding)
+#define PIC_SEG_CELLS 4
+#define PIC_SORT_SLICES 4
 
 struct SortBufs {
   int* keys;        // [ptotal]

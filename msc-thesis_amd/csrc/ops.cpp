@@ -106,7 +106,7 @@ int pic_op_sort(const int32_t* keys, int64_t n, int ncells, int32_t* perm, int32
   std::vector<int2> pit, cit, sit;
   for (long long q = 0; q < std::max<long long>(n, 1); q += PIC_CHUNK) pit.push_back(make_int2(0, (int)q));
   for (int c = 0; c < ncells; c += PIC_CELL_CHUNK) cit.push_back(make_int2(0, c));
-  for (int c = 0; c < ncells; c += 64) sit.push_back(make_int2(0, c));
+  for (int c = 0; c < ncells; c += PIC_SEG_CELLS) sit.push_back(make_int2(0, c));
   d.cell_item_begin = 0;
   d.cell_item_end = (int)cit.size();
   int2 *dp, *dc, *ds;
