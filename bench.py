@@ -273,13 +273,15 @@ def _collide_cpu(cfg, parts, nsteps):
 def dsmc_rate(steps=100, warmup=10):
     """The DSMC workload: 256 x 256, a Gaussian blob of 2,097,152 particles (183 per cell at
     the centre), NTC hard spheres per cell on the cell-sorted state every step (nu dt ~ 0.1
-    at the centre); particle-steps/s of the whole step after warm-up steps."""
+    at the centre); particle-steps/s of the whole step after autotuning and warm-up steps."""
     import torch
     from pic_amd import Simulation, presets
     cfg, parts, _ = presets.dsmc_blob()
     with Simulation(cfg) as sim:
         sim.set_particles(*parts)
         n = sim.num_particles()
+        sim.autotune(4)  # tile / halo / splits (the cell sort runs every step)
+        tune = sim.tuning()
         sim.step(warmup)
         sim.sync()
         torch.cuda.synchronize()
@@ -291,7 +293,7 @@ def dsmc_rate(steps=100, warmup=10):
         c1 = sim.collision_counts()[0]
     return {"particle_steps_per_s": n * steps / dt, "ms_per_step": dt * 1e3 / steps,
             "particles": n, "steps": steps, "grid": [cfg.nx, cfg.ny],
-            "collisions_per_step": (c1 - c0) / steps}
+            "collisions_per_step": (c1 - c0) / steps, "tuning": tune}
 
 
 def collide_cpu(which, nsteps):

@@ -158,7 +158,6 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   }
 }
 
-#define DSMC_WAVES 4  // waves per workgroup; each takes one cell at a time
 
 // Pair selection of a DSMC round: slot t of [0, n) through a 12-round balanced Feistel
 // network on 2*half bits (the smallest even width with 2^(2 half) >= n; round i keyed by
@@ -217,77 +216,159 @@ __device__ __forceinline__ int dsmc_cell_rm(int c, const DsmcArgs& a) {
   return cx * a.ny + cy;
 }
 
-// NTC hard-sphere collisions in rounds of disjoint pairs, one wave per cell.  The cell's
-// M candidates (M = floor(mr) + [u < frac(mr)], mr = 1/2 n (n-1) coeff gmax) are dealt to
-// rounds of P = n/2 pairs; the pairs of a round are disjoint, so its candidates go to the
-// lanes at once and read and write the velocities in HBM directly (only the candidates'
-// particles are touched).  Candidate k is accepted when u gm < g, gm = max(gmax at the
-// round's start, g); an accepted pair keeps its centre-of-mass velocity and turns its
-// relative velocity to a uniform direction.  After a round gmax = max(gmax, every g of
-// the round) (a wave maximum: exact, order-free); a further round first waits for this
-// round's stores (workgroup-scope fence: the lanes of one wave share the CU's L1).
-__global__ __launch_bounds__(64 * DSMC_WAVES) void k_dsmc(DsmcArgs a) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// One candidate: the pair of slots (pi(2m), pi(2m+1)) of the cell's round permutation,
+// accepted when u gm < g with gm = max(gmax at the round's start, g); an accepted pair keeps
+// its centre-of-mass velocity and turns its relative velocity to a uniform direction.
+// Returns g (for the round's new gmax); counts acceptances in acc.
+__device__ __forceinline__ double dsmc_candidate(const DsmcArgs& a, int b, uint32_t n,
+                                                 int half, const uint32_t key[4], int cr,
+                                                 uint32_t s32, long long k, uint32_t m,
+                                                 double gm, unsigned int& acc) {
+  const long long pi = b + (long long)pair_perm(2u * m, n, half, key);
+  const long long pj = b + (long long)pair_perm(2u * m + 1u, n, half, key);
+  double w[4];
+  draw4((uint32_t)cr, s32, (uint32_t)k, PIC_TAG_DSMC, a.seed, w);
+  const double vix = a.vx[pi], viy = a.vy[pi], vjx = a.vx[pj], vjy = a.vy[pj];
+  const double gx = vix - vjx, gy = viy - vjy;
+  const double g = sqrt(gx * gx + gy * gy);
+  const double ge = g > gm ? g : gm;
+  if (w[0] * ge < g) {
+    double sn, cs;
+    sincos2pi_own(w[1], sn, cs);
+    const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
+    const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
+    a.vx[pi] = (float)(cx + hx);
+    a.vy[pi] = (float)(cy + hy);
+    a.vx[pj] = (float)(cx - hx);
+    a.vy[pj] = (float)(cy - hy);
+    ++acc;
+  }
+  return g;
+}
+
+// NTC hard-sphere collisions in rounds of disjoint pairs.  A cell of n particles has
+// M = floor(mr) + [u < frac(mr)] candidates, mr = 1/2 n (n-1) coeff gmax, dealt to rounds of
+// P = n/2 pairs; the pairs of a round are disjoint, so its candidates are independent.
+// After a round gmax = max(gmax, every g of the round) (a maximum: exact, order-free).
+//
+// One workgroup takes DSMC_CELLS consecutive cells: the per-cell set-up (offsets, gmax,
+// the draw of M, the first round's permutation key) runs a thread per cell, then the
+// first-round candidates of all its cells are dealt to the workgroup's threads together
+// (most cells need one round of a few candidates), and the few cells with more rounds go
+// on one at a time with the threads over the round's pairs.  Velocities are read and
+// written in HBM directly (only the candidates' particles are touched); a round that
+// follows another first waits for its stores (__syncthreads: workgroup-scope fence).
+#define DSMC_CELLS 64
+#define DSMC_THREADS 256
+__global__ __launch_bounds__(DSMC_THREADS) void k_dsmc(DsmcArgs a) {
+  __shared__ int s_b[DSMC_CELLS], s_n[DSMC_CELLS], s_cr[DSMC_CELLS], s_half[DSMC_CELLS];
+  __shared__ int s_first[DSMC_CELLS + 1];
+  __shared__ long long s_M[DSMC_CELLS];
+  __shared__ uint32_t s_key[DSMC_CELLS][4];
+  __shared__ double s_gm[DSMC_CELLS];
+  __shared__ unsigned long long s_gr[DSMC_CELLS];  // bits of the largest g (g >= 0: ordered)
+  __shared__ unsigned long long s_more;
+  const int tid = threadIdx.x;
   const long long step = a.st ? a.st->step - 1 : a.step;
   const uint32_t s32 = (uint32_t)step;
   const uint32_t sk0 = (uint32_t)a.seed, sk1 = (uint32_t)(a.seed >> 32);
   unsigned int acc = 0;
-  for (int c = blockIdx.x * DSMC_WAVES + wv; c < a.ncells; c += gridDim.x * DSMC_WAVES) {
-    const int b = a.offsets[c], n = a.offsets[c + 1] - b;
-    if (n < 2) continue;
-    const int cr = dsmc_cell_rm(c, a);
-    double gm = (double)a.gmax[cr];
-    double u[4];
-    draw4((uint32_t)cr, s32, 0xffffffffu, PIC_TAG_DSMC, a.seed, u);
-    const double mr = 0.5 * (double)n * (double)(n - 1) * a.coeff * gm;
-    const double fl = floor(mr);
-    long long M = (long long)fl + ((u[0] < mr - fl) ? 1 : 0);
-    if (M > 0x7fffffffll) M = 0x7fffffffll;
-    if (M == 0) continue;
-    const int P = n >> 1;
-    const int half = pair_half_bits((uint32_t)n);
-    for (long long r0 = 0; r0 < M; r0 += P) {
-      uint32_t key[4] = {(uint32_t)cr, s32, 0x80000000u | (uint32_t)(r0 / P), PIC_TAG_DSMP};
-      philox4x32_10(key, sk0, sk1);
-      const int npr = (int)((M - r0 < P) ? (M - r0) : P);
-      double gr = gm;
-      for (int m = lane; m < npr; m += 64) {
-        const long long pi = b + (long long)pair_perm(2u * m, (uint32_t)n, half, key);
-        const long long pj = b + (long long)pair_perm(2u * m + 1u, (uint32_t)n, half, key);
-        double w[4];
-        draw4((uint32_t)cr, s32, (uint32_t)(r0 + m), PIC_TAG_DSMC, a.seed, w);
-        const double vix = a.vx[pi], viy = a.vy[pi], vjx = a.vx[pj], vjy = a.vy[pj];
-        const double gx = vix - vjx, gy = viy - vjy;
-        const double g = sqrt(gx * gx + gy * gy);
-        const double ge = g > gm ? g : gm;
-        if (g > gr) gr = g;
-        if (w[0] * ge < g) {
-          double sn, cs;
-          sincos2pi_own(w[1], sn, cs);
-          const double cx = 0.5 * (vix + vjx), cy = 0.5 * (viy + vjy);
-          const double hx = 0.5 * g * cs, hy = 0.5 * g * sn;
-          a.vx[pi] = (float)(cx + hx);
-          a.vy[pi] = (float)(cy + hy);
-          a.vx[pj] = (float)(cx - hx);
-          a.vy[pj] = (float)(cy - hy);
-          ++acc;
+  for (int c0 = blockIdx.x * DSMC_CELLS; c0 < a.ncells; c0 += gridDim.x * DSMC_CELLS) {
+    // ---- per-cell set-up, the first wave, a lane per cell
+    if (tid < DSMC_CELLS) {
+      const int c = c0 + tid;
+      int b = 0, n = 0, cr = 0;
+      long long M = 0;
+      double gm = 0.0;
+      if (c < a.ncells) {
+        b = a.offsets[c];
+        n = a.offsets[c + 1] - b;
+        if (n >= 2) {
+          cr = dsmc_cell_rm(c, a);
+          gm = (double)a.gmax[cr];
+          double u[4];
+          draw4((uint32_t)cr, s32, 0xffffffffu, PIC_TAG_DSMC, a.seed, u);
+          const double mr = 0.5 * (double)n * (double)(n - 1) * a.coeff * gm;
+          const double fl = floor(mr);
+          M = (long long)fl + ((u[0] < mr - fl) ? 1 : 0);
+          if (M > 0x7fffffffll) M = 0x7fffffffll;
         }
       }
-      gm = wave_max_f64(gr);
-      if (r0 + P < M) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const int P = n >> 1;
+      const int r1 = M > 0 ? (int)(M < P ? M : P) : 0;  // candidates of the first round
+      int incl = r1;                                    // inclusive scan over the lanes
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (tid >= o) incl += t;
+      }
+      s_b[tid] = b;
+      s_n[tid] = n;
+      s_cr[tid] = cr;
+      s_M[tid] = M;
+      s_gm[tid] = gm;
+      s_gr[tid] = (unsigned long long)__double_as_longlong(gm);
+      s_first[tid + 1] = incl;
+      if (tid == 0) s_first[0] = 0;
+      const unsigned long long more = __ballot(M > (long long)P && r1 > 0);
+      if (tid == 0) s_more = more;
+      if (r1 > 0) {
+        uint32_t key[4] = {(uint32_t)cr, s32, 0x80000000u, PIC_TAG_DSMP};
+        philox4x32_10(key, sk0, sk1);
+        s_key[tid][0] = key[0]; s_key[tid][1] = key[1];
+        s_key[tid][2] = key[2]; s_key[tid][3] = key[3];
+        s_half[tid] = pair_half_bits((uint32_t)n);
       }
     }
-    if (lane == 0) a.gmax[cr] = (float)gm;
+    __syncthreads();
+    // ---- first round of every cell, candidates dealt to the threads
+    const int total = s_first[DSMC_CELLS];
+    for (int t = tid; t < total; t += DSMC_THREADS) {
+      int q = 0;  // cell of candidate t: s_first[q] <= t < s_first[q + 1]
+#pragma unroll
+      for (int w = DSMC_CELLS / 2; w > 0; w >>= 1)
+        if (s_first[q + w] <= t) q += w;
+      const uint32_t m = (uint32_t)(t - s_first[q]);
+      const uint32_t key[4] = {s_key[q][0], s_key[q][1], s_key[q][2], s_key[q][3]};
+      const double g = dsmc_candidate(a, s_b[q], (uint32_t)s_n[q], s_half[q], key, s_cr[q], s32,
+                                      (long long)m, m, s_gm[q], acc);
+      atomicMax(&s_gr[q], (unsigned long long)__double_as_longlong(g));
+    }
+    __syncthreads();
+    // ---- cells with further rounds, one at a time
+    unsigned long long more = s_more;
+    while (more) {
+      const int q = __builtin_ctzll(more);
+      more &= more - 1;
+      const long long Mq = s_M[q];
+      const int nq = s_n[q], Pq = nq >> 1, bq = s_b[q], crq = s_cr[q], hq = s_half[q];
+      for (long long r0 = Pq; r0 < Mq; r0 += Pq) {
+        // the previous round's stores and gmax are in (__syncthreads above / below)
+        const double gq = __longlong_as_double((long long)s_gr[q]);
+        uint32_t key[4] = {(uint32_t)crq, s32, 0x80000000u | (uint32_t)(r0 / Pq), PIC_TAG_DSMP};
+        philox4x32_10(key, sk0, sk1);
+        const int npr = (int)((Mq - r0 < Pq) ? (Mq - r0) : Pq);
+        __syncthreads();  // every thread has read s_gr[q] before it moves
+        for (int m = tid; m < npr; m += DSMC_THREADS) {
+          const double g = dsmc_candidate(a, bq, (uint32_t)nq, hq, key, crq, s32, r0 + m,
+                                          (uint32_t)m, gq, acc);
+          atomicMax(&s_gr[q], (unsigned long long)__double_as_longlong(g));
+        }
+        __syncthreads();
+      }
+    }
+    if (tid < DSMC_CELLS && c0 + tid < a.ncells && s_first[tid + 1] > s_first[tid])
+      a.gmax[s_cr[tid]] = (float)__longlong_as_double((long long)s_gr[tid]);
+    __syncthreads();  // LDS reused by the next group of cells
   }
   // accepted pairs: integer sum, order-free
   unsigned int t = acc;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-  if (lane == 0 && t && a.ncoll) atomicAdd((unsigned long long*)a.ncoll, (unsigned long long)t);
-  if (lane == 0 && t && a.diag) atomicAdd(&a.diag[(step % PIC_DIAG_CAP) * 8 + 6], (double)t);
+  if ((tid & 63) == 0 && t) {
+    if (a.ncoll) atomicAdd((unsigned long long*)a.ncoll, (unsigned long long)t);
+    if (a.diag) atomicAdd(&a.diag[(step % PIC_DIAG_CAP) * 8 + 6], (double)t);
+  }
 }
 
 }  // namespace
@@ -304,7 +385,7 @@ void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scra
 }
 
 void launch_dsmc(hipStream_t s, const DsmcArgs& a) {
-  int blocks = (a.ncells + DSMC_WAVES - 1) / DSMC_WAVES;
-  blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
-  k_dsmc<<<blocks, 64 * DSMC_WAVES, 0, s>>>(a);
+  int blocks = (a.ncells + DSMC_CELLS - 1) / DSMC_CELLS;
+  blocks = blocks < 1 ? 1 : (blocks > 65536 ? 65536 : blocks);
+  k_dsmc<<<blocks, DSMC_THREADS, 0, s>>>(a);
 }
