@@ -106,6 +106,7 @@ Ctx::~Ctx() {
   F(d_segitems); F(d_tscan_items);
   F(partials_push); F(partials_rows); F(diag);
   F(d_coll_counts); F(birth_count); F(birth_parent); F(birth_buf); F(gmax); F(flag);
+  F(coll_list); F(coll_count); F(coll_partials);
   for (auto& e : ev) if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -332,6 +333,11 @@ int Ctx::alloc() {
     PIC_HIP(dalloc(&birth_parent, maxcap));
     PIC_HIP(dalloc(&birth_buf, 4 * maxcap));
     PIC_HIP(hipMemset(birth_count, 0, sizeof(int)));
+    int lb, lp;
+    mcc_list_layout(maxcap, &lb, &lp);
+    PIC_HIP(dalloc(&coll_list, (size_t)lb * lp));
+    PIC_HIP(dalloc(&coll_count, PIC_MCC_BLOCKS));
+    PIC_HIP(dalloc(&coll_partials, 4 * PIC_MCC_BLOCKS));
   }
   if (any_dsmc) PIC_HIP(dalloc(&gmax, ctotal));
   // twiddles exp(-2 pi i j / PIC_FFT_MAX)
@@ -456,10 +462,11 @@ int Ctx::launch_collide_stage() {
       a.seed = c.seed;
       a.counts = d_coll_counts + 4 * r;
       a.birth_count = birth_count; a.birth_parent = birth_parent; a.birth_buf = birth_buf;
+      a.coll_list = coll_list; a.coll_count = coll_count; a.coll_partials = coll_partials;
       a.from_state_step = 1;
       a.diag = diag + d.dbase * 8;
       launch_mcc(stream, a, d.pcap);
-      launch_mcc_births(stream, a, sb.scratch, rho, gsum, d_runs, r);
+      launch_mcc_births(stream, a, sb.scratch, rho, gsum, d_runs, r, d.pcap);
     }
   }
   if (any_dsmc) {

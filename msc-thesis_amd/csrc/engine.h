@@ -81,6 +81,9 @@ struct Ctx {
   // collisions
   long long* d_coll_counts = nullptr;  // [nruns*4]
   int* birth_count = nullptr;
+  int* coll_list = nullptr;   // MCC: slots that pass the null-collision test
+  int* coll_count = nullptr;
+  int* coll_partials = nullptr;
   int* birth_parent = nullptr;
   float* birth_buf = nullptr;
   float* gmax = nullptr;               // per cell (ctotal)

@@ -19,19 +19,58 @@ __device__ __forceinline__ void cross_sections(double e, double& sel, double& se
   siz = (e > 15.8) ? 3e-20 * log_own(e / 15.8) / e : 0.0;
 }
 
-__global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int n_upper) {
-  __shared__ int cnt[4];
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+// Null-collision test, every particle: one Philox4x32-10 block keyed by (id, step); the
+// particles whose first uniform falls below p_coll = 1 - exp(-nu_max dt) (about 1 in 100)
+// go to the step's collision list.  Workgroup b appends its hits to its own region
+// [b * per_block, (b + 1) * per_block) of the list through an LDS counter (one LDS atomic per
+// wave; one global counter for all of them serialised ~60k atomics per step on a single
+// address) and stores its count.  The collisions run on the list in a second launch, so the
+// waves do not carry the ~1 % of lanes through the long FP64 collision branch while the
+// other 63 wait.
+__global__ __launch_bounds__(PIC_WG) void k_mcc_null(MccArgs a, int n_upper, int per_block) {
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   const int np = min(a.st->np, n_upper);
   const long long step = a.from_state_step ? a.st->step - 1 : a.step;
   const uint32_t s32 = (uint32_t)step;
+  const uint32_t sk0 = (uint32_t)a.seed, sk1 = (uint32_t)(a.seed >> 32);
+  const int lane = threadIdx.x & 63;
+  int* list = a.coll_list + (long long)blockIdx.x * per_block;
   for (int q = blockIdx.x * PIC_WG + threadIdx.x; q < np; q += gridDim.x * PIC_WG) {
-    const long long i = a.pbase + q;
+    uint32_t ctr[4] = {(uint32_t)a.id[a.pbase + q], s32, 0u, PIC_TAG_MCC};
+    philox4x32_10(ctr, sk0, sk1);
+    const bool hit = u01(ctr[0]) < a.p_coll;
+    const unsigned long long mask = __ballot(hit);
+    if (mask) {
+      const int leader = __builtin_ctzll(mask);
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s_cnt, (int)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      if (hit) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = q;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.coll_count[blockIdx.x] = s_cnt;
+}
+
+// The collisions of the step's list (workgroup b takes region b of the null test): the same
+// Philox block as the null test (recomputed), the process chosen by u1 nu_max dt against
+// the cumulative nu_k(E) dt, isotropic scattering in the velocity plane; an ionisation
+// appends a birth (ordered by parent id afterwards).
+__global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int per_block) {
+  __shared__ int cnt[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int ncand = a.coll_count[blockIdx.x];
+  const int* list = a.coll_list + (long long)blockIdx.x * per_block;
+  const long long step = a.from_state_step ? a.st->step - 1 : a.step;
+  const uint32_t s32 = (uint32_t)step;
+  for (int k = threadIdx.x; k < ncand; k += PIC_WG) {
+    const long long i = a.pbase + list[k];
     const uint32_t pid = (uint32_t)a.id[i];
     double u[4];
     draw4(pid, s32, 0u, PIC_TAG_MCC, a.seed, u);
-    if (u[0] >= a.p_coll) continue;
     double vx = (double)a.vx[i], vy = (double)a.vy[i];
     const double v2 = vx * vx + vy * vy;
     const double sp = sqrt(v2);
@@ -82,23 +121,42 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int n_upper) {
     a.vy[i] = (float)vy;
   }
   __syncthreads();
-  if (threadIdx.x < 4 && cnt[threadIdx.x] && a.counts)
-    atomicAdd((unsigned long long*)&a.counts[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
-  // collisions of the step (elastic + excitation + ionisation) into its diagnostics row;
-  // integer-valued doubles, so the sum is exact and order-free
-  const int real = cnt[0] + cnt[1] + cnt[2];
-  if (threadIdx.x == 0 && a.diag && real)
-    atomicAdd(&a.diag[(step % PIC_DIAG_CAP) * 8 + 6], (double)real);
+  // the workgroup's counts; k_mcc_births adds them up (thousands of workgroups adding to the
+  // same four counters would serialise on those addresses)
+  if (threadIdx.x < 4) a.coll_partials[4 * blockIdx.x + threadIdx.x] = cnt[threadIdx.x];
 }
 
 // Births of the step: order by parent id (bitonic, one workgroup), assign ids
-// next_id + rank, append after np, deposit their charge into the current parity.
+// next_id + rank, append after np, deposit their charge into the current parity.  First the
+// step's collision counts: the per-workgroup counts of k_mcc added up (into the cumulative
+// counters and the diagnostics row of the step).
 __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long long* scratch,
                                                        unsigned long long* rho,
                                                        unsigned long long* rho_lo_unused,
                                                        long long gsum, const RunDesc* runs,
-                                                       int run) {
+                                                       int run, int nblocks) {
   __shared__ unsigned long long s[2048];
+  __shared__ unsigned long long red[4][4];
+  {
+    unsigned long long c[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += PIC_WG)
+      for (int k = 0; k < 4; ++k) c[k] += (unsigned long long)a.coll_partials[4 * b + k];
+    for (int k = 0; k < 4; ++k)
+      for (int o = 32; o > 0; o >>= 1) c[k] += __shfl_xor(c[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x >> 6] = c[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t[4];
+      for (int k = 0; k < 4; ++k) t[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+      if (a.counts)
+        for (int k = 0; k < 4; ++k) a.counts[k] += (long long)t[k];
+      if (a.diag) {
+        const long long step = a.from_state_step ? a.st->step - 1 : a.step;
+        a.diag[(step % PIC_DIAG_CAP) * 8 + 6] += (double)(t[0] + t[1] + t[2]);
+      }
+    }
+  }
   const int nb = min(*a.birth_count, a.cap);
   if (nb == 0) return;
   int m = 1;
@@ -373,15 +431,27 @@ __global__ __launch_bounds__(DSMC_THREADS) void k_dsmc(DsmcArgs a) {
 
 }  // namespace
 
+void mcc_list_layout(long long n_upper, int* blocks, int* per_block) {
+  long long b = (n_upper + PIC_WG - 1) / PIC_WG;
+  b = b < 1 ? 1 : (b > PIC_MCC_BLOCKS ? PIC_MCC_BLOCKS : b);
+  const long long per = (n_upper + b * PIC_WG - 1) / (b * PIC_WG) * PIC_WG;  // grid-stride share
+  *blocks = (int)b;
+  *per_block = (int)(per < PIC_WG ? PIC_WG : per);
+}
+
 void launch_mcc(hipStream_t s, const MccArgs& a, int n_upper) {
-  int blocks = (n_upper + PIC_WG - 1) / PIC_WG;
-  blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-  k_mcc<<<blocks, PIC_WG, 0, s>>>(a, n_upper);
+  int blocks, per_block;
+  mcc_list_layout(n_upper, &blocks, &per_block);
+  k_mcc_null<<<blocks, PIC_WG, 0, s>>>(a, n_upper, per_block);
+  k_mcc<<<blocks, PIC_WG, 0, s>>>(a, per_block);
 }
 
 void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scratch,
-                       unsigned long long* rho, long long gsum, const RunDesc* runs, int run) {
-  k_mcc_births<<<1, PIC_WG, 0, s>>>(a, scratch, rho, nullptr, gsum, runs, run);
+                       unsigned long long* rho, long long gsum, const RunDesc* runs, int run,
+                       int n_upper) {
+  int blocks, per_block;
+  mcc_list_layout(n_upper, &blocks, &per_block);
+  k_mcc_births<<<1, PIC_WG, 0, s>>>(a, scratch, rho, nullptr, gsum, runs, run, blocks);
 }
 
 void launch_dsmc(hipStream_t s, const DsmcArgs& a) {

@@ -221,10 +221,19 @@ struct MccArgs {
   long long step;
   double* diag;                   // the run's diagnostics rows (may be NULL): the step's
                                   // collisions are added to column PIC_DIAG_NCOLL
+  int* coll_list;                 // slots that pass the null-collision test, a region of
+                                  // per_block entries per workgroup (mcc_list_layout)
+  int* coll_count;                // [blocks] entries of each region
+  int* coll_partials;             // [blocks][4] collision counts of each k_mcc workgroup
 };
+// MCC collision list layout for n_upper particles: the null-test grid and the list region
+// of each workgroup (its grid-stride share); the list needs blocks * per_block entries
+#define PIC_MCC_BLOCKS 2048
+void mcc_list_layout(long long n_upper, int* blocks, int* per_block);
 void launch_mcc(hipStream_t s, const MccArgs& a, int n_upper);
 void launch_mcc_births(hipStream_t s, const MccArgs& a, unsigned long long* scratch,
-                       unsigned long long* rho, long long gsum, const RunDesc* runs, int run);
+                       unsigned long long* rho, long long gsum, const RunDesc* runs, int run,
+                       int n_upper);
 struct DsmcArgs {
   float *vx, *vy;
   const int* offsets;             // [ncells + 1] cell ranges in the sort's numbering

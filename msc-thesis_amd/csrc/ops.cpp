@@ -341,7 +341,7 @@ int pic_op_mcc(float* x, float* y, float* vx, float* vy, int32_t* id, int64_t* n
   DevMem m;
   RunState* st;
   RunDesc* dd;
-  int *bc, *bp;
+  int *bc, *bp, *cl, *cc, *cp;
   float* bb;
   unsigned long long *scratch, *rho;
   PIC_HIP(m.get(&st, 1));
@@ -349,6 +349,11 @@ int pic_op_mcc(float* x, float* y, float* vx, float* vy, int32_t* id, int64_t* n
   PIC_HIP(m.get(&bc, 1));
   PIC_HIP(m.get(&bp, std::max<int64_t>(cap, 1)));
   PIC_HIP(m.get(&bb, 4 * std::max<int64_t>(cap, 1)));
+  int lb, lp;
+  mcc_list_layout(std::max<int64_t>(*n_io, 1), &lb, &lp);
+  PIC_HIP(m.get(&cl, (size_t)lb * lp));
+  PIC_HIP(m.get(&cc, PIC_MCC_BLOCKS));
+  PIC_HIP(m.get(&cp, 4 * PIC_MCC_BLOCKS));
   PIC_HIP(m.get(&scratch, 2 * std::max<int64_t>(cap, 1)));
   PIC_HIP(m.get(&rho, 2 * (size_t)c.nx * c.ny));
   RunState hs;
@@ -376,10 +381,11 @@ int pic_op_mcc(float* x, float* y, float* vx, float* vy, int32_t* id, int64_t* n
   a.seed = c.seed;
   a.counts = (long long*)counts;
   a.birth_count = bc; a.birth_parent = bp; a.birth_buf = bb;
+  a.coll_list = cl; a.coll_count = cc; a.coll_partials = cp;
   a.from_state_step = 0;
   a.step = step;
   launch_mcc(s, a, (int)*n_io);
-  launch_mcc_births(s, a, scratch, rho, (long long)c.nx * c.ny, dd, 0);
+  launch_mcc_births(s, a, scratch, rho, (long long)c.nx * c.ny, dd, 0, (int)*n_io);
   PIC_HIP(hipGetLastError());
   PIC_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
   PIC_HIP(hipStreamSynchronize(s));
