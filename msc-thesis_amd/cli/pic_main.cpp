@@ -13,6 +13,8 @@
 //   -N STEPS   number of steps (overrides -t)           -p f32|f64  particle precision
 //   -r SEED    Philox key of the loaders and collisions -a        autotune before timing
 //   -q PPC     particles per cell of the landau / twostream / mcc presets
+//   -o FILE    final particles (CSV x,y,vx,vy in input order, synchronous velocities): the
+//              positions the reference returns from picFloatGpu (xResult, yResult)
 //   -h         this text
 // The reference preset loads the reference's lattice and Maxwellian stream
 // (pic_load_reference); the others are the configured workloads of the engine.
@@ -40,7 +42,7 @@ void usage() {
   std::printf(
       "usage: pic_main [-x XDIM] [-y YDIM] [-l N] [-n NPX] [-m NPY] [-t TMAX] [-f FILE]\n"
       "                [-s BLOCK] [-c] [-g] [-P reference|landau|twostream|mcc] [-d DT]\n"
-      "                [-N STEPS] [-p f32|f64] [-r SEED] [-q PPC] [-a] [-h]\n");
+      "                [-N STEPS] [-p f32|f64] [-r SEED] [-q PPC] [-a] [-o FILE] [-h]\n");
 }
 
 bool check(int rc, const char* what) {
@@ -57,14 +59,14 @@ int main(int argc, char** argv) {
   // reference defaults (CLd/inc/params.h:21-47)
   int xdim = 10, ydim = 10, ngrid = 32, npx = 10, npy = 10;
   double tmax = 1e-4, dt = -1.0;
-  std::string file, preset = "reference", prec = "f64";
+  std::string file, final_file, preset = "reference", prec = "f64";
   bool gpu = true, autotune = false;
   long long nsteps = -1;
   uint64_t seed = 1234;
   int ppc = 64;
   bool grid_set = false, dim_set = false;
   int opt;
-  while ((opt = getopt(argc, argv, "x:y:l:n:m:t:f:s:cgP:d:N:p:r:q:ah")) != -1) {
+  while ((opt = getopt(argc, argv, "x:y:l:n:m:t:f:s:cgP:d:N:p:r:q:ao:h")) != -1) {
     switch (opt) {
       case 'x': xdim = std::atoi(optarg); dim_set = true; break;
       case 'y': ydim = std::atoi(optarg); dim_set = true; break;
@@ -83,6 +85,7 @@ int main(int argc, char** argv) {
       case 'r': seed = std::strtoull(optarg, nullptr, 10); break;
       case 'q': ppc = std::atoi(optarg); break;
       case 'a': autotune = true; break;
+      case 'o': final_file = optarg; break;
       case 'h': usage(); return 0;
       default: usage(); return 1;
     }
@@ -213,6 +216,24 @@ int main(int argc, char** argv) {
     done += k;
   }
   if (out) std::fclose(out);
+  if (ok && !final_file.empty()) {
+    int64_t nf = 0;
+    ok = check(pic_num_particles(ctx, 0, &nf), "pic_num_particles");
+    std::vector<double> fx(nf), fy(nf), fvx(nf), fvy(nf);
+    ok = ok && check(pic_get_particles(ctx, 0, fx.data(), fy.data(), fvx.data(), fvy.data(), nf, 1),
+                     "pic_get_particles");
+    FILE* fo = ok ? std::fopen(final_file.c_str(), "w") : nullptr;
+    if (ok && !fo) {
+      std::fprintf(stderr, "Error: cannot open %s\n", final_file.c_str());
+      ok = false;
+    }
+    if (fo) {
+      std::fprintf(fo, "x,y,vx,vy\n");
+      for (int64_t i = 0; i < nf; ++i)
+        std::fprintf(fo, "%.17g,%.17g,%.17g,%.17g\n", fx[i], fy[i], fvx[i], fvy[i]);
+      std::fclose(fo);
+    }
+  }
   int tile = 0, halo = 0, splits = 0, se = 0;
   pic_get_tuning(ctx, &tile, &halo, &splits, &se);
   pic_destroy(ctx);

@@ -21,6 +21,7 @@ def run(*args, timeout=120):
 def test_help():
     r = run("-h")
     assert r.returncode == 0 and "-P reference|landau|twostream|mcc" in r.stdout
+    assert "[-o FILE]" in r.stdout
 
 
 @pytest.mark.parametrize("args", [["-l", "48"], ["-n", "0"], ["-x", "-3"], ["-p", "f16"],
@@ -62,3 +63,28 @@ def test_landau_run_matches_context_api(tmp_path):
     want = np.genfromtxt(ref, delimiter=",", names=True)
     for name in want.dtype.names:
         assert np.array_equal(got[name], want[name]), name
+
+
+@pytest.mark.gpu
+def test_reference_defaults_match_the_oracle(tmp_path, oracle):
+    """pic_main with the reference driver's defaults (thesisCode/collisionless/double/src/
+    main.cc:41-72, inc/params.h:21-47: 10 x 10 domain, 10 x 10 particle lattice, tmax 1e-4,
+    dt 1e-7, so 1000 steps, FP64; the grid is 32^2, the default power of two) loaded by the
+    reference's own initialiser (main.cc:201-244, pic_load_reference): the final positions
+    and velocities it writes (-o) against the oracle's FP64 run from the same initial state
+    (or_init_reference + or_run_f64)."""
+    out = tmp_path / "final.csv"
+    r = run("-o", str(out))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "1000 steps" in r.stdout and "100 particles" in r.stdout
+    got = np.genfromtxt(out, delimiter=",", names=True)
+    n = 100
+    x, y, vx, vy = (np.zeros(n) for _ in range(4))
+    oracle.or_init_reference(10, 10, 10, 10, 32, 32, 1.0, 1.0, 0.0, 0.0,
+                             *(oracle.p(a) for a in (x, y, vx, vy)))
+    x0 = x.copy()
+    oracle.run_f64(x, y, vx, vy, 32, 32, 10.0, 10.0, 1e-7, -1.0, -(10.0 * 10.0) / n, 1000)
+    assert np.abs(x - x0).max() > 0  # the particles moved
+    # fixed-point CIC weights (2^-30 of a cell) against the oracle's continuous weights
+    for name, want in zip(("x", "y", "vx", "vy"), (x, y, vx, vy)):
+        assert np.abs(got[name] - want).max() <= 1e-12 * max(1.0, np.abs(want).max()), name
