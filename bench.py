@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
-    ap.add_argument("--no-mcc", action="store_true", help="skip the collision workload")
+    ap.add_argument("--no-mcc", action="store_true", help="skip the collision workloads")
+    ap.add_argument("--no-two-stream", action="store_true", help="skip the FP64 two-stream leg")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--stream", action="store_true",
@@ -118,20 +119,28 @@ def stream_probe(reps=20, n=1 << 24):
     in-place x += 1 passes over 16,777,216 FP32 values each (the x, y, vx, vy arrays of the
     benched workload: 536.9 MB read + written), timed with HIP events, after a 1 GiB fill
     that evicts them from the L2 and the last-level cache (cold), and repeated without the
-    fill (warm: partly served from the last-level cache)."""
+    fill (warm: partly served from the last-level cache); then the same passes out of place,
+    each array read and x + 1 written to a second array."""
     import torch
     arrs = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(4)]
+    dsts = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(4)]
     flush = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     out = {}
-    for mode in ("cold", "warm"):
+    # in place (the plain push's pattern) and out of place into a second set of arrays (the
+    # sort step's pattern: read one buffer, write the other), same bytes
+    for mode in ("cold", "warm", "cold_out_of_place", "warm_out_of_place"):
         ms = []
         for r in range(reps):
-            if mode == "cold":
+            if mode.startswith("cold"):
                 flush.fill_(float(r))
             e0.record()
-            for a in arrs:
-                a.add_(1.0)
+            if mode.endswith("out_of_place"):
+                for a, d in zip(arrs, dsts):
+                    torch.add(a, 1.0, out=d)
+            else:
+                for a in arrs:
+                    a.add_(1.0)
             e1.record()
             torch.cuda.synchronize()
             ms.append(e0.elapsed_time(e1))
@@ -482,6 +491,9 @@ def main():
                 line["dsmc_256"]["cpu_baseline"] = collide_cpu("dsmc", 3)
         except Exception as e:
             line["dsmc_256"] = {"error": repr(e)}
+    if args.no_two_stream:
+        print(json.dumps(line))
+        return
     try:
         line["two_stream_64"] = two_stream_rate()
         if not args.no_cpu:

@@ -91,21 +91,31 @@ def test_mcc_matches_oracle_bitwise(oracle):
 
 
 def test_mcc_rates_within_3_sigma():
+    """Counts per process summed over 8 seeded passes (different particles, seeds and step
+    indices) within 3 sigma of sum p_coll nu_k(E) dt / (nu_max dt)."""
     cfg = mcc_cfg()
-    n = 400_000
-    arrs = maxwellian_with_tail(n, 2)
-    _, _, cnt = run_mcc_op(arrs, n + n // 2, cfg, step=0)
     p = mcc_params(cfg)
-    v2 = arrs[2].astype(np.float64) ** 2 + arrs[3].astype(np.float64) ** 2
-    e = p["e_per_v2"] * v2
-    sp = np.sqrt(v2)
-    sel = 1e-19 / (1 + e / 10)
-    sex = np.where(e > 11.5, 2e-21 * (1 - 11.5 / np.maximum(e, 1e-30)), 0)
-    siz = np.where(e > 15.8, 3e-20 * np.log(np.maximum(e, 15.8) / 15.8) / np.maximum(e, 1e-30), 0)
-    for k, s in enumerate((sel, sex, siz)):
-        prob = p["p_coll"] * p["nu_dt"] * s * sp / p["numax_dt"]
-        mu, sd = prob.sum(), math.sqrt((prob * (1 - prob)).sum())
-        assert abs(cnt[k] - mu) <= 3 * sd + 1, (k, cnt[k], mu, sd)
+    n = 200_000
+    tot = np.zeros(3)
+    mu = np.zeros(3)
+    var = np.zeros(3)
+    for seed in range(8):
+        arrs = maxwellian_with_tail(n, 20 + seed)
+        cfg.seed = 1000 + seed
+        _, _, cnt = run_mcc_op(arrs, n + n // 2, cfg, step=seed)
+        v2 = arrs[2].astype(np.float64) ** 2 + arrs[3].astype(np.float64) ** 2
+        e = p["e_per_v2"] * v2
+        sp = np.sqrt(v2)
+        sel = 1e-19 / (1 + e / 10)
+        sex = np.where(e > 11.5, 2e-21 * (1 - 11.5 / np.maximum(e, 1e-30)), 0)
+        siz = np.where(e > 15.8, 3e-20 * np.log(np.maximum(e, 15.8) / 15.8) / np.maximum(e, 1e-30), 0)
+        for k, s_ in enumerate((sel, sex, siz)):
+            prob = p["p_coll"] * p["nu_dt"] * s_ * sp / p["numax_dt"]
+            mu[k] += prob.sum()
+            var[k] += (prob * (1 - prob)).sum()
+        tot += cnt[:3]
+    for k in range(3):
+        assert abs(tot[k] - mu[k]) <= 3 * math.sqrt(var[k]), (k, tot[k], mu[k])
 
 
 def test_mcc_energy_bookkeeping():
