@@ -491,8 +491,22 @@ int Ctx::launch_collide_stage() {
   return PIC_OK;
 }
 
-int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev) {
-  launch_field_stage(0, ev);
+int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
+  {
+    FieldArgs fa = field_args(0);
+    if (fin_pending) {
+      fa.fold_finish = 1;
+      fa.ppush = partials_push;
+    }
+    if (scan_pending) {
+      fa.fold_scan = 1;
+      fa.ftable = sb.ftable; fa.fchunk_sums = sb.fchunk_sums; fa.fstride = sb.max_buckets;
+      fa.fscan_items = sb.fscan_items; fa.nfscan_items = sb.nfscan_items;
+      fa.ftoff = scan_toff;
+    }
+    launch_field(stream, fa, ev);
+    fin_pending = scan_pending = false;
+  }
   // fused sort: the push one step before the sort step counts the tiles of the positions
   // it produces (then the counts are scanned), the push of the sort step scatters
   int smode = 0;
@@ -502,10 +516,16 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev) {
   }
   launch_push_stage(0, smode);
   if (ev) (void)hipEventRecord(ev[7], stream);  // the push stage is the k_push launch alone
-  launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
+  if (defer_tail) fin_pending = true;
+  else launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
   if (smode == 1) {
     sb.toff = toffs[1 - cur];
-    launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
+    if (defer_tail) {
+      scan_pending = true;
+      scan_toff = toffs[1 - cur];
+    } else {
+      launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
+    }
   }
   if (ev) (void)hipEventRecord(ev[4], stream);
   launch_collide_stage();
@@ -530,10 +550,15 @@ int Ctx::capture_macro(int which) {
   const int saved_cur = cur, saved_since = steps_since_sort;
   hipGraph_t g = nullptr;
   PIC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < k; ++i) launch_one_step(true);
+  // inside the sequence each step's end-of-step work runs in the next step's field kernels
+  // (the collision stages read the step counter after the finish: not with collisions);
+  // the last step ends with its own kernels, so a replay starts and ends like eager steps
+  const bool fold = !any_mcc && !any_dsmc;
+  for (int i = 0; i < k; ++i) launch_one_step(true, nullptr, fold && i + 1 < k);
   hipError_t e = hipStreamEndCapture(stream, &g);
   cur = saved_cur;
   steps_since_sort = saved_since;
+  fin_pending = scan_pending = false;
   if (e != hipSuccess) return fail(PIC_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&macro[which], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);

@@ -115,8 +115,14 @@ struct Ctx {
   int launch_sort_stage();
   int launch_collide_stage();
   double numax_dt_of(int r);
-  // one step; ev (7 events or NULL) brackets the stages for pic_stage_times
-  int launch_one_step(bool allow_sort, hipEvent_t* ev = nullptr);
+  // one step; ev (7 events or NULL) brackets the stages for pic_stage_times.  defer_tail:
+  // the step's end-of-step work (diagnostics row and step counter; the count-table scan on
+  // a count step) is left to the next step's field kernels (fold_finish / fold_scan of
+  // FieldArgs), which must then be launched next, by launch_one_step
+  int launch_one_step(bool allow_sort, hipEvent_t* ev = nullptr, bool defer_tail = false);
+  bool fin_pending = false;   // the previous step's finish is deferred
+  bool scan_pending = false;  // the previous step's count-table scan is deferred
+  int* scan_toff = nullptr;   // its tile offsets output
   int step(int nsteps);
   int capture_macro(int which);
 };

@@ -13,9 +13,14 @@
 //   field   : Ex + i Ey from one inverse complex row FFT per row (groups of rows per
 //             workgroup); writes the padded (Ex, Ey) node array and the field energy
 //             (fixed-order reduction)
+// Inside a captured sequence of steps the three kernels also run the previous step's
+// end-of-step work in extra workgroups after their own items (step_tail.h): its
+// diagnostics row (rows) and step counter (columns), and on the step after a count step
+// the fused sort's count-table scan (rows, columns, field).
 #include <type_traits>
 #include "fft_lds.h"
 #include "launch.h"
+#include "step_tail.h"
 
 namespace {
 
@@ -34,7 +39,20 @@ __device__ __forceinline__ int ilog2(int v) { return 31 - __clz(v); }
 // consecutive threads on consecutive rows.
 __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   extern __shared__ double2 lds[];
-  const int2 it = a.row_pair_items[xcd_item(blockIdx.x, gridDim.x)];
+  if (blockIdx.x >= a.n_row_pair_items) {
+    // extra workgroups: the previous step's diagnostics rows, then the count-table sums
+    int g = blockIdx.x - a.n_row_pair_items;
+    if (a.fold_finish) {
+      if (g < a.nruns) {
+        step_finish_run(a.runs, a.st, a.ppush, a.partials, a.diag, g, false);
+        return;
+      }
+      g -= a.nruns;
+    }
+    fscan_part(a.runs, a.ftable, a.fchunk_sums, a.fstride, a.fscan_items, g);
+    return;
+  }
+  const int2 it = a.row_pair_items[xcd_item(blockIdx.x, a.n_row_pair_items)];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
   const int PP = rd.fft_pp;
@@ -50,7 +68,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
           make_double2(r[(long long)(2 * q) * ny + j], r[(long long)(2 * q + 1) * ny + j]);
     }
   } else {
-    const int par = (int)(a.st[it.x].step & 1);
+    const int par = (int)((a.st[it.x].step + a.fold_finish) & 1);
     const unsigned long long* rh = a.rho + par * a.gsum;
     const int zpar = a.clear_read ? par : 1 - par;
     unsigned long long* zw = a.rho_w + zpar * a.gsum;
@@ -82,6 +100,10 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
 
 __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
   extern __shared__ double2 lds[];
+  if (blockIdx.x >= a.n_col_items) {  // extra workgroups: the count-table scan, one per run
+    fscan_base(a.runs, a.st, a.fchunk_sums, a.fstride, a.ftoff, blockIdx.x - a.n_col_items);
+    return;
+  }
   const int2 it = a.col_items[blockIdx.x];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, lognx = rd.lognx;
@@ -96,11 +118,18 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
   for (int i = threadIdx.x; i < nx; i += PIC_WG) col[bitrev(i, lognx)] = S[i];
   __syncthreads();
   fft_lds_rows(col, 1, nx, lognx, -1, twl, lognx);
-  if (k == 0 && threadIdx.x == 0 && a.diag && !a.rho_in) {
-    const double2 r1 = col[1];
+  if (k == 0 && threadIdx.x == 0 && !a.rho_in) {
+    // the one reader and writer of the run's step counter in this kernel
     const long long stp = a.st[it.x].step;
-    a.diag[(rd.dbase + (stp % PIC_DIAG_CAP)) * 8 + 4] =
-        sqrt(r1.x * r1.x + r1.y * r1.y) / ((double)nx * ny);
+    if (a.diag) {
+      const double2 r1 = col[1];
+      a.diag[(rd.dbase + ((stp + a.fold_finish) % PIC_DIAG_CAP)) * 8 + 4] =
+          sqrt(r1.x * r1.x + r1.y * r1.y) / ((double)nx * ny);
+    }
+    if (a.fold_finish) {  // the previous step's end (its row came from the rows pass)
+      a.st[it.x].step = stp + 1;
+      a.st[it.x].vsync = 0;
+    }
   }
   const double twopi = 6.283185307179586476925;
   const int kk = (2 * k < ny) ? k : k - ny;
@@ -138,7 +167,12 @@ template <typename R, int MODE>
 __global__ __launch_bounds__(PIC_WG) void k_fft_field(FieldArgs a) {
   extern __shared__ double2 lds[];
   __shared__ double red[4];
-  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int nitems = (MODE == 0) ? a.n_row_items : a.n_row_pair_items;
+  if (blockIdx.x >= nitems) {  // extra workgroups (MODE 0): the count-table slots
+    fscan_rows(a.runs, a.ftable, a.fchunk_sums, a.fstride, a.fscan_items, blockIdx.x - nitems);
+    return;
+  }
+  const int item = xcd_item(blockIdx.x, nitems);
   const int2 it = (MODE == 0) ? a.row_items[item] : a.row_pair_items[item];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, logny = rd.logny;
@@ -261,15 +295,18 @@ void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev) {
   const size_t lrow = ((size_t)a.row_lds + a.max_ny / 2) * sizeof(double2);
   const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3) +
                       (size_t)(a.max_nx / 2) * sizeof(double2);
+  // extra workgroups of the folded end-of-step work
+  const int xrows = (a.fold_finish ? a.nruns : 0) + (a.fold_scan ? a.nfscan_items : 0);
+  const int xcols = a.fold_scan ? a.nruns : 0, xfield = a.fold_scan ? a.nfscan_items : 0;
   if (ev) (void)hipEventRecord(ev[0], s);
-  k_fft_rows<<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);
+  k_fft_rows<<<a.n_row_pair_items + xrows, PIC_WG, lrow, s>>>(a);
   if (ev) (void)hipEventRecord(ev[1], s);
-  k_fft_cols<<<a.n_col_items, PIC_WG, lcol, s>>>(a);
+  k_fft_cols<<<a.n_col_items + xcols, PIC_WG, lcol, s>>>(a);
   if (ev) (void)hipEventRecord(ev[2], s);
   if (a.fp64)
-    k_fft_field<double, 0><<<a.n_row_items, PIC_WG, lrow, s>>>(a);
+    k_fft_field<double, 0><<<a.n_row_items + xfield, PIC_WG, lrow, s>>>(a);
   else
-    k_fft_field<float, 0><<<a.n_row_items, PIC_WG, lrow, s>>>(a);
+    k_fft_field<float, 0><<<a.n_row_items + xfield, PIC_WG, lrow, s>>>(a);
   if (a.want_phi && a.phi) k_fft_field<float, 1><<<a.n_row_pair_items, PIC_WG, lrow, s>>>(a);
   if (ev) (void)hipEventRecord(ev[3], s);
 }

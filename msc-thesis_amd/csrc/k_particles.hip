@@ -11,6 +11,7 @@
 #include <type_traits>
 #include "launch.h"
 #include "deposit.h"
+#include "step_tail.h"
 
 namespace {
 
@@ -25,32 +26,6 @@ __device__ __forceinline__ int engine_cell_key(int cx, int cy, const RunDesc& rd
   const int sh = __builtin_ctz((unsigned)rd.tile), m = rd.tile - 1;
   const int t = (cx >> sh) * rd.tiles_y + (cy >> sh);
   return (((t << sh) + (cx & m)) << sh) + (cy & m);
-}
-
-// In-place exclusive scan of a[0..n) by one 256-thread workgroup (tiles of 256 values
-// with a carried prefix); returns the total in every thread.
-__device__ int block_exclusive_scan(int* a, int n, int* wsum) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int carry = 0;
-  for (int base = 0; base < n; base += PIC_WG) {
-    const int i = base + threadIdx.x;
-    const int v = (i < n) ? a[i] : 0;
-    int inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += t;
-    }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    int wpre = 0;
-    for (int k = 0; k < w; ++k) wpre += wsum[k];
-    const int tile_total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    if (i < n) a[i] = carry + wpre + inc - v;
-    carry += tile_total;
-    __syncthreads();
-  }
-  return carry;
 }
 
 template <typename R, bool FROM_POS>
@@ -489,86 +464,19 @@ __global__ __launch_bounds__(PIC_WG) void k_tscan_add(const RunDesc* __restrict_
   }
 }
 
-// ---- scan of the fused sort's count table.  Row-major: row c (one item of the push grid)
-// holds that item's count for every tile, so each push workgroup reads and writes one
-// contiguous row.  The destination slot of (tile b, row c) is
-//   base[b] + sum_{c' < c} count[c'][b],   base[b] = sum_{b' < b} sum_c count[c][b'],
-// i.e. tile-major order with rows in item order.  Rows are scanned in groups of
-// FSCAN_ROWS: k_fscan_part sums each group's columns, k_fscan_base scans the group sums
-// down each column and the column totals across tiles (one workgroup per run; it also
-// writes the tile offsets of the new order), k_fscan_rows writes the slots.
-
-// FSCAN_UNROLL independent loads per thread are issued before any of them is used
-#define FSCAN_UNROLL 32
-
+// ---- scan of the fused sort's count table (the workgroup bodies are in step_tail.h)
 __global__ __launch_bounds__(PIC_WG) void k_fscan_part(const RunDesc* __restrict__ runs,
                                                        const int* __restrict__ table,
                                                        int* __restrict__ part, int stride,
                                                        const int2* __restrict__ items) {
-  const int2 it = items[blockIdx.x];
-  const RunDesc rd = runs[it.x];
-  const int nb = rd.nbuckets;
-  const int r0 = it.y * FSCAN_ROWS, r1 = min(r0 + FSCAN_ROWS, rd.fchunks);
-  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
-    const int* col = table + rd.ftbase + b;
-    int sum = 0;
-    for (int c0 = r0; c0 < r1; c0 += FSCAN_UNROLL) {
-      int v[FSCAN_UNROLL];
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k)
-        v[k] = (c0 + k < r1) ? col[(long long)(c0 + k) * nb] : 0;
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k) sum += v[k];
-    }
-    part[(long long)blockIdx.x * stride + b] = sum;
-  }
+  fscan_part(runs, table, part, stride, items, blockIdx.x);
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_fscan_base(const RunDesc* __restrict__ runs,
                                                        RunState* __restrict__ st,
                                                        int* __restrict__ part, int stride,
                                                        int* __restrict__ toff) {
-  __shared__ int tot[512];  // one per tile (the fused sort allows at most 512)
-  __shared__ int wsum[4];
-  const RunDesc rd = runs[blockIdx.x];
-  if (rd.fchunks <= 0) return;  // uniform: this run sorts with the separate kernels
-  const int nb = rd.nbuckets, g0 = rd.fscan_item_begin, g1 = rd.fscan_item_end;
-  // column totals over the row groups
-  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
-    int sum = 0;
-    for (int gi = g0; gi < g1; gi += FSCAN_UNROLL) {
-      int v[FSCAN_UNROLL];
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k)
-        v[k] = (gi + k < g1) ? part[(long long)(gi + k) * stride + b] : 0;
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k) sum += v[k];
-    }
-    tot[b] = sum;
-  }
-  __syncthreads();
-  block_exclusive_scan(tot, nb, wsum);
-  // each group's first slot per tile: the tile base plus the groups before it
-  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
-    int run = tot[b];
-    toff[rd.toffbase + b] = run;
-    for (int gi = g0; gi < g1; gi += FSCAN_UNROLL) {
-      int v[FSCAN_UNROLL];
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k)
-        v[k] = (gi + k < g1) ? part[(long long)(gi + k) * stride + b] : 0;
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k) {
-        if (gi + k < g1) part[(long long)(gi + k) * stride + b] = run;
-        run += v[k];
-      }
-    }
-  }
-  if (threadIdx.x == 0) {
-    const int np = st[blockIdx.x].np;
-    toff[rd.toffbase + nb] = np;
-    st[blockIdx.x].nsorted = np;
-  }
+  fscan_base(runs, st, part, stride, toff, blockIdx.x);
 }
 
 __global__ __launch_bounds__(PIC_WG) void k_fscan_rows(const RunDesc* __restrict__ runs,
@@ -576,25 +484,7 @@ __global__ __launch_bounds__(PIC_WG) void k_fscan_rows(const RunDesc* __restrict
                                                        const int* __restrict__ part,
                                                        int stride,
                                                        const int2* __restrict__ items) {
-  const int2 it = items[blockIdx.x];
-  const RunDesc rd = runs[it.x];
-  const int nb = rd.nbuckets;
-  const int r0 = it.y * FSCAN_ROWS, r1 = min(r0 + FSCAN_ROWS, rd.fchunks);
-  for (int b = threadIdx.x; b < nb; b += PIC_WG) {
-    int* col = table + rd.ftbase + b;
-    int run = part[(long long)blockIdx.x * stride + b];
-    for (int c0 = r0; c0 < r1; c0 += FSCAN_UNROLL) {
-      int v[FSCAN_UNROLL];
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k)
-        v[k] = (c0 + k < r1) ? col[(long long)(c0 + k) * nb] : 0;
-#pragma unroll
-      for (int k = 0; k < FSCAN_UNROLL; ++k) {
-        if (c0 + k < r1) col[(long long)(c0 + k) * nb] = run;
-        run += v[k];
-      }
-    }
-  }
+  fscan_rows(runs, table, part, stride, items, blockIdx.x);
 }
 
 template <typename R>
@@ -1044,57 +934,14 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
   sort_tail(s, runs, st, nullptr, sb, citems, ncitems);
 }
 
-// End of a step, one workgroup per run: the diagnostics row of the step from the
-// per-workgroup partial sums of the field pass (field energy, in row-group order) and of
-// the push (kinetic energy and momentum, in item order), then the step counter.  Kernel
-// order makes the partials visible (no grid-wide fences in the producers).
+// End of a step, one workgroup per run: the diagnostics row, then the step counter.
+// Kernel order makes the partials visible (no grid-wide fences in the producers).
 __global__ __launch_bounds__(PIC_WG) void k_step_finish(const RunDesc* __restrict__ runs,
                                                         RunState* __restrict__ st,
                                                         const double* __restrict__ ppush,
                                                         const double* __restrict__ prows,
                                                         double* __restrict__ diag) {
-  __shared__ double red[4][4];
-  const int run = blockIdx.x;
-  // the step counter and particle count do not depend on the sums: loaded first, so their
-  // latency overlaps the partials'
-  long long stp = 0;
-  int np = 0;
-  if (threadIdx.x == 0) {
-    stp = st[run].step;
-    np = st[run].np;
-  }
-  const RunDesc rd = runs[run];
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = rd.push_item_begin + threadIdx.x; i < rd.push_item_end; i += PIC_WG) {
-    v[0] += ppush[3 * (long long)i + 0];
-    v[1] += ppush[3 * (long long)i + 1];
-    v[2] += ppush[3 * (long long)i + 2];
-  }
-  for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) v[3] += prows[q];
-  // the four workgroup sums in one pass (each: wave sums, then the four waves in order)
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0)
-    for (int k = 0; k < 4; ++k) red[k][w] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double s0 = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-    const double s1 = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
-    const double s2 = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
-    const double s3 = ((red[3][0] + red[3][1]) + red[3][2]) + red[3][3];
-    RunState* sr = st + run;
-    const double mp = rd.qp / rd.qm;
-    double* row = diag + (rd.dbase + (stp % PIC_DIAG_CAP)) * 8;
-    row[0] = 0.5 * s3 * rd.dx * rd.dy;
-    row[1] = 0.5 * mp * s0;
-    row[2] = mp * s1;
-    row[3] = mp * s2;
-    row[5] = (double)np;
-    row[6] = 0.0;  // the collision stage of this step adds its count
-    sr->step = stp + 1;
-    sr->vsync = 0;
-  }
+  step_finish_run(runs, st, ppush, prows, diag, blockIdx.x, true);
 }
 
 void launch_step_finish(hipStream_t s, const RunDesc* runs, RunState* st, int nruns,

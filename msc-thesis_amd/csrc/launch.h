@@ -189,6 +189,20 @@ struct FieldArgs {
   const double* rho_in;              // op mode: FP64 density input instead of rho (may be NULL)
   int clear_read;                    // 1: zero the charge words that are read (one buffer);
                                      // 0: zero the other parity's (the next push's target)
+  // End-of-step work of the previous step run by extra workgroups of these kernels
+  // (step_tail.h).  fold_finish: the rows pass writes the previous step's diagnostics row
+  // (nruns workgroups after its row items), and the column pass advances the step counter
+  // (so every kernel of this solve reads the counter one behind and adds fold_finish).
+  // fold_scan: the rows, column and field passes run the fused sort's count-table scan
+  // (fscan_part, fscan_base, fscan_rows) after their own items.
+  int fold_finish;
+  const double* ppush;               // push partials of the previous step (fold_finish)
+  int fold_scan;
+  int* ftable;
+  int* fchunk_sums;
+  int fstride;
+  const int2* fscan_items; int nfscan_items;
+  int* ftoff;                        // tile offsets of the order the next push writes
 };
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev /* 4 events or NULL */);
 void launch_op_efield(hipStream_t s, const double* phi, double* ex, double* ey, int nx, int ny,
