@@ -365,6 +365,42 @@ __device__ __forceinline__ int bucket_lead(R x, R y, R vx, R vy, const Geo<R>& g
   return bucket_of<R>(wrap_pos(fmar(t, vx, x), g.Lx), wrap_pos(fmar(t, vy, y), g.Ly), g, rd);
 }
 
+// The same key with its place relative to tile (tx, ty), for the scatter step's ranking:
+// bits 0-15 the key, bits 16 and up the neighbour code (dx + 1) * 3 + (dy + 1) of a key
+// tile within one tile of (tx, ty) along each axis (periodically), else 9.  With fewer than
+// 3 tiles along an axis the neighbours are not distinct tiles: every key gets 9.
+#define PUSH_NCODES 9
+template <typename R>
+__device__ __forceinline__ int bucket_lead_code(R x, R y, R vx, R vy, const Geo<R>& g,
+                                                const RunDesc& rd, int tx, int ty) {
+  if (rd.sort_lead != 0.0) {
+    const R t = (R)rd.sort_lead;
+    x = wrap_pos(fmar(t, vx, x), g.Lx);
+    y = wrap_pos(fmar(t, vy, y), g.Ly);
+  }
+  int cx, cy;
+  R f;
+  locate_w(x, g.idx, g.nx, cx, f);
+  locate_w(y, g.idy, g.ny, cy, f);
+  const int sh = __builtin_ctz((unsigned)rd.tile);
+  const int kx = cx >> sh, ky = cy >> sh;
+  int dx = kx - tx, dy = ky - ty;
+  if (dx > 1) dx -= rd.tiles_x;
+  else if (dx < -1) dx += rd.tiles_x;
+  if (dy > 1) dy -= rd.tiles_y;
+  else if (dy < -1) dy += rd.tiles_y;
+  const bool near = rd.tiles_x >= 3 && rd.tiles_y >= 3 && dx >= -1 && dx <= 1 && dy >= -1 &&
+                    dy <= 1;
+  const int code = near ? (dx + 1) * 3 + (dy + 1) : PUSH_NCODES;
+  return (kx * rd.tiles_y + ky) | (code << 16);
+}
+
+// lanes below this one with their bit set in m
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 template <typename R>
 __global__ __launch_bounds__(PIC_WG) void k_tile_hist(const RunDesc* __restrict__ runs,
                                                       const RunState* __restrict__ st,
