@@ -395,6 +395,14 @@ __device__ __forceinline__ int bucket_lead_code(R x, R y, R vx, R vy, const Geo<
   return (kx * rd.tiles_y + ky) | (code << 16);
 }
 
+// the tile of neighbour code c of tile (tx, ty) (bucket_lead_code)
+__device__ __forceinline__ int code_tile(int c, int tx, int ty, const RunDesc& rd) {
+  int kx = tx + c / 3 - 1, ky = ty + c % 3 - 1;
+  kx = kx < 0 ? kx + rd.tiles_x : (kx >= rd.tiles_x ? kx - rd.tiles_x : kx);
+  ky = ky < 0 ? ky + rd.tiles_y : (ky >= rd.tiles_y ? ky - rd.tiles_y : ky);
+  return kx * rd.tiles_y + ky;
+}
+
 // lanes below this one with their bit set in m
 __device__ __forceinline__ int lanes_below(unsigned long long m) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
