@@ -566,13 +566,14 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
     int vid[SCAT_U], key[SCAT_U];
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {
+      // lanes past p1 load particle p1 - 1 (p1 > q0 here): loads without branches
       const int p = q0 + 64 * u + lane;
-      const bool act = p < p1;
-      vx[u] = act ? src.x[pb + p] : (R)0;
-      vy[u] = act ? src.y[pb + p] : (R)0;
-      vvx[u] = act ? src.vx[pb + p] : (R)0;
-      vvy[u] = act ? src.vy[pb + p] : (R)0;
-      vid[u] = act ? src.id[pb + p] : 0;
+      const int pl = p < p1 ? p : p1 - 1;
+      vx[u] = src.x[pb + pl];
+      vy[u] = src.y[pb + pl];
+      vvx[u] = src.vx[pb + pl];
+      vvy[u] = src.vy[pb + pl];
+      vid[u] = src.id[pb + pl];
     }
 #pragma unroll
     for (int u = 0; u < SCAT_U; ++u) {

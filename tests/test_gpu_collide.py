@@ -52,13 +52,13 @@ def maxwellian_with_tail(n, seed):
     return [a.astype(np.float32) for a in (x, y, vx, vy)]
 
 
-def run_mcc_op(arrs, cap, cfg, step):
+def run_mcc_op(arrs, cap, cfg, step, ids=None):
     n = arrs[0].size
     ts = [torch.zeros(cap, dtype=torch.float32, device=DEV) for _ in range(4)]
     for t, a in zip(ts, arrs):
         t[:n] = torch.from_numpy(a)
     tid = torch.zeros(cap, dtype=torch.int32, device=DEV)
-    tid[:n] = torch.arange(n, dtype=torch.int32)
+    tid[:n] = torch.arange(n, dtype=torch.int32) if ids is None else torch.from_numpy(ids)
     cnt = torch.zeros(4, dtype=torch.int64, device=DEV)
     nio = ctypes.c_int64(n)
     check(lib.pic_op_mcc(*(dptr(t) for t in ts), dptr(tid), ctypes.byref(nio), cap,
@@ -85,6 +85,32 @@ def test_mcc_matches_oracle_bitwise(oracle):
                                cfg.seed, 5, oracle.p(rcnt))
     assert np.array_equal(gcnt, rcnt)
     assert gx.size == m and rcnt[2] == m - n > 0
+    for g, r in zip((gx, gy, gvx, gvy), ref):
+        assert np.array_equal(g, r[:m])
+    assert np.array_equal(gid, rid[:m])
+
+
+def test_mcc_with_permuted_ids_matches_oracle(oracle):
+    """The draws are keyed by particle id, not by slot: particles in a shuffled order, with
+    their ids, give the oracle's result on the same arrays (births appended in parent-id
+    order)."""
+    cfg = mcc_cfg()
+    n, cap = 120_001, 160_000
+    arrs = maxwellian_with_tail(n, 4)
+    ids = np.random.default_rng(5).permutation(n).astype(np.int32)
+    (gx, gy, gvx, gvy), gid, gcnt = run_mcc_op(arrs, cap, cfg, step=11, ids=ids)
+    p = mcc_params(cfg)
+    ref = [np.zeros(cap, np.float32) for _ in range(4)]
+    for r, a in zip(ref, arrs):
+        r[:n] = a
+    rid = np.zeros(cap, np.int32)
+    rid[:n] = ids
+    rcnt = np.zeros(4, np.int64)
+    m = oracle.or_mcc_step_f32(n, cap, *(oracle.p(r) for r in ref), oracle.p(rid), n,
+                               p["e_per_v2"], p["nu_dt"], p["p_coll"], p["numax_dt"], p["m2"],
+                               cfg.seed, 11, oracle.p(rcnt))
+    assert np.array_equal(gcnt, rcnt) and rcnt[:3].sum() > 0
+    assert gx.size == m
     for g, r in zip((gx, gy, gvx, gvy), ref):
         assert np.array_equal(g, r[:m])
     assert np.array_equal(gid, rid[:m])
