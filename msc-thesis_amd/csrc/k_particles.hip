@@ -51,22 +51,46 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict_
   const int nx = rd.nx, ny = rd.ny;
   const R dxr = (R)rd.dx, dyr = (R)rd.dy;
   int* cnt = counts + rd.cbase;
+  // a slice is RANK_U particles per thread: their loads go out together (lanes past p1 load
+  // particle p1 - 1, no branches) and their keys stay in registers for the later passes
+  constexpr int RANK_U = (PIC_CHUNK / PIC_SORT_SLICES) / PIC_WG;
+  static_assert(RANK_U * PIC_WG == PIC_CHUNK / PIC_SORT_SLICES, "slice = whole rounds");
+  int key[RANK_U];
+  bool act[RANK_U];
   int kmin = INT_MAX, kmax = INT_MIN;
-  for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) {
-    int key;
-    if (FROM_POS) {
+  if (FROM_POS) {
+    R xv[RANK_U], yv[RANK_U];
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u) {
+      const int p = p0 + threadIdx.x + u * PIC_WG;
+      act[u] = p < p1;
+      const int pl = act[u] ? p : p1 - 1;
+      xv[u] = x[pb + pl];
+      yv[u] = y[pb + pl];
+    }
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u) {
       int cx, cy;
       R f;
-      locate(x[pb + p], dxr, nx, cx, f);
-      locate(y[pb + p], dyr, ny, cy, f);
-      key = engine_cell_key(cx, cy, rd);
-      keys[pb + p] = key;
-    } else {
-      key = keys_in[pb + p];
+      locate(xv[u], dxr, nx, cx, f);
+      locate(yv[u], dyr, ny, cy, f);
+      key[u] = engine_cell_key(cx, cy, rd);
+      if (act[u]) keys[pb + p0 + threadIdx.x + u * PIC_WG] = key[u];
     }
-    kmin = min(kmin, key);
-    kmax = max(kmax, key);
+  } else {
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u) {
+      const int p = p0 + threadIdx.x + u * PIC_WG;
+      act[u] = p < p1;
+      key[u] = keys_in[pb + (act[u] ? p : p1 - 1)];
+    }
   }
+#pragma unroll
+  for (int u = 0; u < RANK_U; ++u)
+    if (act[u]) {
+      kmin = min(kmin, key[u]);
+      kmax = max(kmax, key[u]);
+    }
   kmin = wave_min_i(kmin);
   kmax = wave_max_i(kmax);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -74,23 +98,27 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict_
   __syncthreads();
   kmin = min(min(red[0], red[1]), min(red[2], red[3]));
   kmax = max(max(red[4], red[5]), max(red[6], red[7]));
-  const int* kv = FROM_POS ? keys : keys_in;
   const long long span = (long long)kmax - kmin + 1;
   if (span <= HWIN) {
     for (int k = threadIdx.x; k < span; k += PIC_WG) hist[k] = 0;
     __syncthreads();
-    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG)
-      rank[pb + p] = atomicAdd(&hist[kv[pb + p] - kmin], 1);
+    int rk[RANK_U];
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u)
+      if (act[u]) rk[u] = atomicAdd(&hist[key[u] - kmin], 1);
     __syncthreads();
     for (int k = threadIdx.x; k < span; k += PIC_WG) {
       int c = hist[k];
       if (c) hist[k] = atomicAdd(&cnt[kmin + k], c);
     }
     __syncthreads();
-    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG) rank[pb + p] += hist[kv[pb + p] - kmin];
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u)
+      if (act[u]) rank[pb + p0 + threadIdx.x + u * PIC_WG] = rk[u] + hist[key[u] - kmin];
   } else {
-    for (int p = p0 + threadIdx.x; p < p1; p += PIC_WG)
-      rank[pb + p] = atomicAdd(&cnt[kv[pb + p]], 1);
+#pragma unroll
+    for (int u = 0; u < RANK_U; ++u)
+      if (act[u]) rank[pb + p0 + threadIdx.x + u * PIC_WG] = atomicAdd(&cnt[key[u]], 1);
   }
 }
 
@@ -318,13 +346,30 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_apply(const RunDesc* __restrict
   const int d0 = it.y + (blockIdx.x % PIC_SORT_SLICES) * (PIC_CHUNK / PIC_SORT_SLICES);
   const int p1 = min(d0 + PIC_CHUNK / PIC_SORT_SLICES, np);
   const long long pb = rd.pbase;
-  for (int d = d0 + threadIdx.x; d < p1; d += PIC_WG) {
-    const long long s = pb + perm[pb + d];
-    dst.x[pb + d] = src.x[s];
-    dst.y[pb + d] = src.y[s];
-    dst.vx[pb + d] = src.vx[s];
-    dst.vy[pb + d] = src.vy[s];
-    dst.id[pb + d] = src.id[s];
+  // APPLY_U slots per thread per round, every load of a round issued before any store (one
+  // slot per thread and round waited for its permutation entry, then for its five reads);
+  // lanes past p1 read slot p1 - 1 (loads without branches) and store nothing
+  constexpr int APPLY_U = 4;
+  for (int d = d0 + threadIdx.x; d < p1; d += APPLY_U * PIC_WG) {
+    int sl[APPLY_U];
+#pragma unroll
+    for (int u = 0; u < APPLY_U; ++u) sl[u] = perm[pb + min(d + u * PIC_WG, p1 - 1)];
+    R x[APPLY_U], y[APPLY_U], vx[APPLY_U], vy[APPLY_U];
+    int idv[APPLY_U];
+#pragma unroll
+    for (int u = 0; u < APPLY_U; ++u) {
+      const long long s = pb + sl[u];
+      x[u] = src.x[s]; y[u] = src.y[s]; vx[u] = src.vx[s]; vy[u] = src.vy[s];
+      idv[u] = src.id[s];
+    }
+#pragma unroll
+    for (int u = 0; u < APPLY_U; ++u) {
+      const int dd = d + u * PIC_WG;
+      if (dd < p1) {
+        dst.x[pb + dd] = x[u]; dst.y[pb + dd] = y[u]; dst.vx[pb + dd] = vx[u];
+        dst.vy[pb + dd] = vy[u]; dst.id[pb + dd] = idv[u];
+      }
+    }
   }
 }
 
