@@ -259,23 +259,47 @@ __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restr
       continue;
     }
     if (n > 64) {
+      // 65-256 entries: a bitonic network over 256 keys held four per lane (key e at lane
+      // e % 64 of register e / 64; padding ~0 sorts last).  Distances of 64 and more pair
+      // registers of one lane, shorter ones lanes of one register.  (Ranking each key by
+      // counting the smaller ones took n steps per wave: most of the sort's time in the
+      // blob's core cells of 100-183 particles.)
       unsigned long long k4[4];
-      int r4[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int j = lane + 64 * k;
         k4[k] = (j < n) ? seg_key(id, pb, perm[pb + b + j]) : ~0ull;
       }
-      for (int jj = 0; jj < n; ++jj) {
-        const int src = jj >> 6;
-        const unsigned long long sel = src == 0 ? k4[0] : (src == 1 ? k4[1] : (src == 2 ? k4[2] : k4[3]));
-        const unsigned long long kj = shfl_u64(sel, jj & 63);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) r4[k] += (kj < k4[k]) ? 1 : 0;
+      for (int size = 2; size <= 256; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          if (stride >= 64) {
+            const int sr = stride >> 6;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              if (k & sr) continue;  // k is the lower element of the pair (k, k | sr)
+              const bool up = ((64 * k) & size) == 0;
+              const unsigned long long a0 = k4[k], a1 = k4[k | sr];
+              const unsigned long long mn = a0 < a1 ? a0 : a1, mx = a0 < a1 ? a1 : a0;
+              k4[k] = up ? mn : mx;
+              k4[k | sr] = up ? mx : mn;
+            }
+          } else {
+            const bool lower = (lane & stride) == 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const unsigned long long o = shfl_xor_u64(k4[k], stride);
+              const bool up = ((lane + 64 * k) & size) == 0;
+              const unsigned long long mn = k4[k] < o ? k4[k] : o, mx = k4[k] < o ? o : k4[k];
+              k4[k] = (lower == up) ? mn : mx;
+            }
+          }
+        }
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (lane + 64 * k < n) perm[pb + b + r4[k]] = (int)(unsigned)(k4[k] & 0xffffffffull);
+        if (lane + 64 * k < n) perm[pb + b + lane + 64 * k] = (int)(unsigned)(k4[k] & 0xffffffffull);
       continue;
     }
     unsigned long long v = (lane < n) ? seg_key(id, pb, perm[pb + b + lane]) : ~0ull;
