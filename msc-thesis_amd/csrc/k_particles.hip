@@ -233,6 +233,51 @@ __device__ __forceinline__ unsigned long long seg_key(const int* id, long long p
 // network across lanes, <= 256 entries by counting ranks (4 entries per lane, keys are
 // unique); larger cells are queued for k_segsort_big.  Items cover PIC_SEG_CELLS cells
 // (one per wave).
+// 65-256 entries of one cell, sorted by one wave: a bitonic network over 64 * NR keys held
+// NR per lane (key e at lane e % 64 of register e / 64; padding ~0 sorts last).  Distances of
+// 64 and more pair registers of one lane, shorter ones lanes of one register.  (Ranking each
+// key by counting the smaller ones took n steps per wave.)
+template <int NR>
+__device__ __forceinline__ void bitonic_regs(const int* __restrict__ id, int* __restrict__ perm,
+                                             long long pb, int b, int n, int lane) {
+  unsigned long long k4[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int j = lane + 64 * k;
+    k4[k] = (j < n) ? seg_key(id, pb, perm[pb + b + j]) : ~0ull;
+  }
+#pragma unroll
+  for (int size = 2; size <= 64 * NR; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= 64) {
+        const int sr = stride >> 6;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          if (k & sr) continue;  // k is the lower element of the pair (k, k | sr)
+          const bool up = ((64 * k) & size) == 0;
+          const unsigned long long a0 = k4[k], a1 = k4[k | sr];
+          const unsigned long long mn = a0 < a1 ? a0 : a1, mx = a0 < a1 ? a1 : a0;
+          k4[k] = up ? mn : mx;
+          k4[k | sr] = up ? mx : mn;
+        }
+      } else {
+        const bool lower = (lane & stride) == 0;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+          const unsigned long long o = shfl_xor_u64(k4[k], stride);
+          const bool up = ((lane + 64 * k) & size) == 0;
+          const unsigned long long mn = k4[k] < o ? k4[k] : o, mx = k4[k] < o ? o : k4[k];
+          k4[k] = (lower == up) ? mn : mx;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k)
+    if (lane + 64 * k < n) perm[pb + b + lane + 64 * k] = (int)(unsigned)(k4[k] & 0xffffffffull);
+}
+
 __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restrict__ runs,
                                                           const int* __restrict__ offsets,
                                                           int* __restrict__ perm,
@@ -258,52 +303,18 @@ __global__ __launch_bounds__(PIC_WG) void k_segsort_small(const RunDesc* __restr
       }
       continue;
     }
+    if (n > 128) {
+      bitonic_regs<4>(id, perm, pb, b, n, lane);
+      continue;
+    }
     if (n > 64) {
-      // 65-256 entries: a bitonic network over 256 keys held four per lane (key e at lane
-      // e % 64 of register e / 64; padding ~0 sorts last).  Distances of 64 and more pair
-      // registers of one lane, shorter ones lanes of one register.  (Ranking each key by
-      // counting the smaller ones took n steps per wave: most of the sort's time in the
-      // blob's core cells of 100-183 particles.)
-      unsigned long long k4[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int j = lane + 64 * k;
-        k4[k] = (j < n) ? seg_key(id, pb, perm[pb + b + j]) : ~0ull;
-      }
-#pragma unroll
-      for (int size = 2; size <= 256; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-          if (stride >= 64) {
-            const int sr = stride >> 6;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              if (k & sr) continue;  // k is the lower element of the pair (k, k | sr)
-              const bool up = ((64 * k) & size) == 0;
-              const unsigned long long a0 = k4[k], a1 = k4[k | sr];
-              const unsigned long long mn = a0 < a1 ? a0 : a1, mx = a0 < a1 ? a1 : a0;
-              k4[k] = up ? mn : mx;
-              k4[k | sr] = up ? mx : mn;
-            }
-          } else {
-            const bool lower = (lane & stride) == 0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const unsigned long long o = shfl_xor_u64(k4[k], stride);
-              const bool up = ((lane + 64 * k) & size) == 0;
-              const unsigned long long mn = k4[k] < o ? k4[k] : o, mx = k4[k] < o ? o : k4[k];
-              k4[k] = (lower == up) ? mn : mx;
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (lane + 64 * k < n) perm[pb + b + lane + 64 * k] = (int)(unsigned)(k4[k] & 0xffffffffull);
+      bitonic_regs<2>(id, perm, pb, b, n, lane);
       continue;
     }
     unsigned long long v = (lane < n) ? seg_key(id, pb, perm[pb + b + lane]) : ~0ull;
-    for (int k = 2; k <= 64; k <<= 1) {
+    // the network over the first m = 2^ceil(log2 n) lanes only (the others hold padding)
+    const int m = 1 << (32 - __clz(n - 1));
+    for (int k = 2; k <= m; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         unsigned long long o = shfl_xor_u64(v, j);
         bool up = (lane & k) == 0;
