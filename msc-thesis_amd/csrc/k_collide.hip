@@ -19,14 +19,17 @@ __device__ __forceinline__ void cross_sections(double e, double& sel, double& se
   siz = (e > 15.8) ? 3e-20 * log_own(e / 15.8) / e : 0.0;
 }
 
-// Null-collision test, every particle: one Philox4x32-10 block keyed by (id, step); the
-// particles whose first uniform falls below p_coll = 1 - exp(-nu_max dt) (about 1 in 100)
-// go to the step's collision list.  Workgroup b appends its hits to its own region
+// Null-collision test, every particle: word 0 of Philox2x32-10 with counter (id, step) and
+// key seed_lo ^ seed_hi ^ 'MCC1'; the particles whose uniform falls below p_coll = 1 -
+// exp(-nu_max dt) (about 1 in 100) go to the step's collision list.  (A Philox4x32-10 block
+// per particle, of which the test used one word, made the kernel VALU-bound at ~100
+// instructions per particle.)  Workgroup b appends its hits to its own region
 // [b * per_block, (b + 1) * per_block) of the list through an LDS counter (one LDS atomic per
 // wave; one global counter for all of them serialised ~60k atomics per step on a single
 // address) and stores its count.  The collisions run on the list in a second launch, so the
 // waves do not carry the ~1 % of lanes through the long FP64 collision branch while the
 // other 63 wait.
+#define MCC_NULL_U 8
 __global__ __launch_bounds__(PIC_WG) void k_mcc_null(MccArgs a, int n_upper, int per_block) {
   __shared__ int s_cnt;
   if (threadIdx.x == 0) s_cnt = 0;
@@ -34,28 +37,40 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_null(MccArgs a, int n_upper, int
   const int np = min(a.st->np, n_upper);
   const long long step = a.from_state_step ? a.st->step - 1 : a.step;
   const uint32_t s32 = (uint32_t)step;
-  const uint32_t sk0 = (uint32_t)a.seed, sk1 = (uint32_t)(a.seed >> 32);
+  const uint32_t nk = (uint32_t)a.seed ^ (uint32_t)(a.seed >> 32) ^ PIC_TAG_MCC;
   const int lane = threadIdx.x & 63;
   int* list = a.coll_list + (long long)blockIdx.x * per_block;
-  for (int q = blockIdx.x * PIC_WG + threadIdx.x; q < np; q += gridDim.x * PIC_WG) {
-    uint32_t ctr[4] = {(uint32_t)a.id[a.pbase + q], s32, 0u, PIC_TAG_MCC};
-    philox4x32_10(ctr, sk0, sk1);
-    const bool hit = u01(ctr[0]) < a.p_coll;
-    const unsigned long long mask = __ballot(hit);
-    if (mask) {
-      const int leader = __builtin_ctzll(mask);
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&s_cnt, (int)__popcll(mask));
-      base = __shfl(base, leader, 64);
-      if (hit) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = q;
+  // MCC_NULL_U ids per thread loaded together (the last id for lanes past the end: always in
+  // bounds), then tested in grid-stride order: one load per test exposed the load latency
+  // at every step of the loop.
+  const int stride = gridDim.x * PIC_WG;
+  const int* __restrict__ ids = a.id + a.pbase;
+  for (int q0 = blockIdx.x * PIC_WG + threadIdx.x; q0 < np; q0 += MCC_NULL_U * stride) {
+    uint32_t idv[MCC_NULL_U];
+#pragma unroll
+    for (int j = 0; j < MCC_NULL_U; ++j) idv[j] = (uint32_t)ids[min(q0 + j * stride, np - 1)];
+#pragma unroll
+    for (int j = 0; j < MCC_NULL_U; ++j) {
+      const int q = q0 + j * stride;
+      uint32_t ctr[2] = {idv[j], s32};
+      philox2x32_10(ctr, nk);
+      const bool hit = q < np && u01(ctr[0]) < a.p_coll;
+      const unsigned long long mask = __ballot(hit);
+      if (mask) {
+        const int leader = __builtin_ctzll(mask);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&s_cnt, (int)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (hit) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = q;
+      }
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) a.coll_count[blockIdx.x] = s_cnt;
 }
 
-// The collisions of the step's list (workgroup b takes region b of the null test): the same
-// Philox block as the null test (recomputed), the process chosen by u1 nu_max dt against
+// The collisions of the step's list (workgroup b takes region b of the null test): a
+// Philox4x32-10 block keyed by (id, step), the process chosen by u1 nu_max dt against
 // the cumulative nu_k(E) dt, isotropic scattering in the velocity plane; an ionisation
 // appends a birth (ordered by parent id afterwards).
 __global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int per_block) {

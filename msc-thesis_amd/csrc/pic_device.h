@@ -152,6 +152,19 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
   }
 }
 
+// Philox2x32-10: one product per round (a single v_mad_u64_u32), half the work of the
+// four-word generator; the MCC null-collision test needs one uniform per particle.
+__device__ __forceinline__ void philox2x32_10(uint32_t c[2], uint32_t k) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) k += 0x9E3779B9u;
+    const uint64_t p = (uint64_t)c[0] * 0xD256D193u;
+    const uint32_t n0 = (uint32_t)(p >> 32) ^ k ^ c[1];
+    c[1] = (uint32_t)p;
+    c[0] = n0;
+  }
+}
+
 #define PIC_TAG_LOAD 0x4c4f4144u
 #define PIC_TAG_MCC  0x4d434331u
 #define PIC_TAG_DSMC 0x44534d43u

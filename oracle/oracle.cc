@@ -56,6 +56,20 @@ void philox(const uint32_t in[4], const uint32_t k[2], uint32_t out[4]) {
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// Philox2x32-10 (same paper): one 32 x 32 product per round, multiplier 0xD256D193, the
+// key bumped by 0x9E3779B9 between rounds.
+void philox2(const uint32_t in[2], uint32_t k, uint32_t out[2]) {
+  uint32_t c0 = in[0], c1 = in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) k += 0x9E3779B9u;
+    uint32_t hi, lo;
+    mulhilo(0xD256D193u, c0, &hi, &lo);
+    c0 = hi ^ k ^ c1;
+    c1 = lo;
+  }
+  out[0] = c0; out[1] = c1;
+}
+
 // Stream tags (4th counter word).
 enum : uint32_t { TAG_LOAD = 0x4c4f4144u, TAG_MCC = 0x4d434331u, TAG_DSMC = 0x44534d43u };
 
@@ -469,6 +483,9 @@ extern "C" {
 
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   philox(ctr, key, out);
+}
+void or_philox2x32_10(const uint32_t ctr[2], const uint32_t key[1], uint32_t out[2]) {
+  philox2(ctr, key[0], out);
 }
 
 uint64_t or_minstd_rand0_nth(uint32_t seed, int n) {
@@ -895,9 +912,13 @@ int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, f
   std::vector<Birth> births;
   const uint32_t s32 = (uint32_t)step;
   for (int64_t p = 0; p < n; ++p) {
-    double u[4];
+    // null test: word 0 of Philox2x32-10 {id, step}, key seed_lo ^ seed_hi ^ 'MCC1'
+    const uint32_t nc[2] = {(uint32_t)id[p], s32};
+    uint32_t nu[2];
+    philox2(nc, (uint32_t)seed ^ (uint32_t)(seed >> 32) ^ TAG_MCC, nu);
+    if (u01(nu[0]) >= p_coll) continue;
+    double u[4];  // the collision's uniforms: u[1] process, u[2] angle, u[3] energy split
     draw4((uint32_t)id[p], s32, 0u, TAG_MCC, seed, u);
-    if (u[0] >= p_coll) continue;
     double vxd = (double)vx[p], vyd = (double)vy[p];
     const double v2 = vxd * vxd + vyd * vyd;
     const double sp = std::sqrt(v2);

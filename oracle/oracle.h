@@ -25,6 +25,7 @@
  *                                    self-force, reference particle.cc via _ref
  *   whole step ..................... Landau damping rate, two-stream growth
  *   Philox4x32-10 .................. Random123 known-answer vectors
+ *   Philox2x32-10 .................. Random123 known-answer vectors
  *   std engines .................... ISO C++ [rand.predef] 10000th values
  *   MCC / DSMC ..................... closed-form rates, conservation laws
  *   adjoint ........................ dot-product tests, central FD
@@ -40,6 +41,8 @@ extern "C" {
 /* ---------------- random numbers ---------------- */
 /* Philox4x32-10 (Salmon et al., SC'11). */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Philox2x32-10 (same paper; key[0] is its single 32-bit key word). */
+void or_philox2x32_10(const uint32_t ctr[2], const uint32_t key[1], uint32_t out[2]);
 /* n-th output (1-based) of std::minstd_rand0 / std::mt19937 seeded with `seed`. */
 uint64_t or_minstd_rand0_nth(uint32_t seed, int n);
 uint64_t or_mt19937_nth(uint32_t seed, int n);
@@ -144,7 +147,10 @@ void or_run_f32_threads(int64_t n, float* x, float* y, float* vx, float* vy,
 void or_mcc_cross_sections(double e_ev, double* s_el, double* s_ex, double* s_iz);
 /* Null-collision MCC pass (mirror of the GPU operator, FP32 particles).  e_per_v2: energy
  * [eV] per |v|^2; nu_dt_per_sigma_v: nu*dt per (sigma [m^2] * |v|); p_coll = 1 - exp(-nu_max
- * dt); mass_ratio2 = 2 m_e / M.  Particle p draws Philox {id[p], step, 0|1, 'MCC1'}.  Births
+ * dt); mass_ratio2 = 2 m_e / M.  Particle p collides when word 0 of Philox2x32-10 {id[p],
+ * step} (key seed_lo ^ seed_hi ^ 'MCC1') gives a uniform below p_coll; a colliding particle
+ * then draws Philox4x32-10 {id[p], step, 0|1, 'MCC1'} (u[1] process, u[2] angle, u[3]
+ * energy split; the second block for the secondary's direction).  Births
  * (one per ionisation) are appended after n in ascending parent id with ids next_id, ...
  * Returns the new particle count; counts[4] += {elastic, excitation, ionisation, null}. */
 int64_t or_mcc_step_f32(int64_t n, int64_t cap, float* x, float* y, float* vx, float* vy,

@@ -21,6 +21,7 @@ _U32P = ctypes.POINTER(ctypes.c_uint32)
 
 SIG = {
     "or_philox4x32_10": (None, [_U32P, _U32P, _U32P]),
+    "or_philox2x32_10": (None, [_U32P, _U32P, _U32P]),
     "or_minstd_rand0_nth": (ctypes.c_uint64, [ctypes.c_uint32, _I]),
     "or_mt19937_nth": (ctypes.c_uint64, [ctypes.c_uint32, _I]),
     "or_init_reference": (None, [_I, _I, _I, _I, _I, _I, _D, _D, _D, _D, _DP, _DP, _DP, _DP]),

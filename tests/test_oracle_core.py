@@ -28,6 +28,17 @@ def test_philox4x32_10_known_answers(oracle, ctr, key, expect):
     assert tuple(int(v) for v in out) == expect
 
 
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0), 0, (0xFF1DAE59, 0x6CD10DF2)),
+    ((0xFFFFFFFF, 0xFFFFFFFF), 0xFFFFFFFF, (0x2C3F628B, 0xAB4FD7AD)),
+])
+def test_philox2x32_10_known_answers(oracle, ctr, key, expect):
+    """Random123 known-answer vectors for Philox2x32-10 (the MCC null-collision draw)."""
+    out = np.zeros(2, np.uint32)
+    oracle.or_philox2x32_10(oracle.p(u32(*ctr)), oracle.p(u32(key)), oracle.p(out))
+    assert tuple(int(v) for v in out) == expect
+
+
 def test_std_engine_known_answers(oracle):
     """ISO C++ [rand.predef]: the 10000th output of a default-constructed engine."""
     assert oracle.or_minstd_rand0_nth(1, 10000) == 1043618065
