@@ -319,6 +319,8 @@ int Ctx::alloc() {
   PIC_HIP(dalloc(&sb.rank, ptotal));
   PIC_HIP(dalloc(&sb.perm, ptotal));
   PIC_HIP(dalloc(&sb.counts, ctotal));
+  // the cell sorts keep the counts zero between sorts (their scan clears what it read)
+  PIC_HIP(hipMemset(sb.counts, 0, sizeof(int) * (size_t)ctotal));
   PIC_HIP(dalloc(&sb.offsets, ctotal));
   PIC_HIP(dalloc(&sb.big_list, 2 * ctotal));
   PIC_HIP(dalloc(&sb.big_count, 1));
@@ -546,7 +548,7 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
 }
 
 int Ctx::capture_macro(int which) {
-  const int k = any_dsmc ? 1 : sort_every;
+  const int k = macro_len();
   const int saved_cur = cur, saved_since = steps_since_sort;
   hipGraph_t g = nullptr;
   PIC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -556,6 +558,7 @@ int Ctx::capture_macro(int which) {
   const bool fold = !any_mcc && !any_dsmc;
   for (int i = 0; i < k; ++i) launch_one_step(true, nullptr, fold && i + 1 < k);
   hipError_t e = hipStreamEndCapture(stream, &g);
+  const int flip = cur ^ saved_cur;  // the sorts inside switch buffers (one per sort)
   cur = saved_cur;
   steps_since_sort = saved_since;
   fin_pending = scan_pending = false;
@@ -564,19 +567,21 @@ int Ctx::capture_macro(int which) {
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return fail(PIC_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
   macro_k[which] = k;
+  macro_flip[which] = flip;
   return PIC_OK;
 }
 
 int Ctx::step(int n) {
-  const int k = any_dsmc ? 1 : sort_every;
+  const int k = macro_len();
   while (n > 0) {
     if (use_graphs && steps_since_sort == 0 && n >= k) {
-      if (!macro[cur]) {
-        int rc = capture_macro(cur);
+      const int w = cur;
+      if (!macro[w]) {
+        int rc = capture_macro(w);
         if (rc) return rc;
       }
-      PIC_HIP(hipGraphLaunch(macro[cur], stream));
-      cur ^= 1;
+      PIC_HIP(hipGraphLaunch(macro[w], stream));
+      cur ^= macro_flip[w];
       n -= k;
     } else {
       launch_one_step(true);
@@ -1122,8 +1127,9 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
         x.launch_sort_stage();
         x.steps_since_sort = 0;
         // one warm cycle (captures graphs), then the timed cycles
-        if ((rc = x.step(se))) break;
-        const int n = std::max(trial_steps / se, 1) * se;
+        const int ml = x.macro_len();
+        if ((rc = x.step(ml))) break;
+        const int n = std::max(trial_steps / ml, 1) * ml;
         PIC_HIP(hipEventRecord(x.ev[0], x.stream));
         if ((rc = x.step(n))) break;
         PIC_HIP(hipEventRecord(x.ev[1], x.stream));

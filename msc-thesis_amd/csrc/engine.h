@@ -92,6 +92,7 @@ struct Ctx {
   // graphs
   hipGraphExec_t macro[2] = {nullptr, nullptr};
   int macro_k[2] = {0, 0};
+  int macro_flip[2] = {0, 0};  // 1: the sequence ends in the other particle buffer
   hipEvent_t ev[8] = {};
 
   ~Ctx();
@@ -125,6 +126,8 @@ struct Ctx {
   int* scan_toff = nullptr;   // its tile offsets output
   int step(int nsteps);
   int capture_macro(int which);
+  // steps per captured sequence: a sort cycle, or (DSMC: a cell sort every step) eight steps
+  int macro_len() const { return any_dsmc ? 8 : sort_every; }
 };
 
 }  // namespace pic

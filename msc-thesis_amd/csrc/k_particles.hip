@@ -122,12 +122,15 @@ __global__ __launch_bounds__(PIC_WG) void k_sort_rank(const RunDesc* __restrict_
   }
 }
 
-// exclusive scan of the counts of one chunk of PIC_CELL_CHUNK cells
+// exclusive scan of the counts of one chunk of PIC_CELL_CHUNK cells; with `clear` the counts
+// are zeroed once read (the engine's sorts keep them zero between sorts, so no fill launch
+// precedes the rank kernel)
 __global__ __launch_bounds__(PIC_WG) void k_scan_local(const RunDesc* __restrict__ runs,
-                                                       const int* __restrict__ counts,
+                                                       int* __restrict__ counts,
                                                        int* __restrict__ offsets,
                                                        int* __restrict__ chunk_sums,
-                                                       const int2* __restrict__ items) {
+                                                       const int2* __restrict__ items,
+                                                       int clear) {
   __shared__ int wsum[4];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
@@ -139,6 +142,11 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_local(const RunDesc* __restrict
     int c = c0 + k;
     v[k] = (c < G) ? counts[rd.cbase + c] : 0;
     s += v[k];
+  }
+  if (clear) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c0 + k < G) counts[rd.cbase + c0 + k] = 0;
   }
   // inclusive wave scan of per-thread totals
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -162,11 +170,15 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_local(const RunDesc* __restrict
   if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
 }
 
-// one workgroup per run: exclusive scan of the chunk sums, total -> offsets[G], nsorted
+// one workgroup per run: exclusive scan of the chunk sums, total -> offsets[G] (and the
+// closing tile offset when toff is given), nsorted; the first workgroup also clears the
+// count of cells too large for the within-cell sort's small kernel (appended to later)
 __global__ __launch_bounds__(PIC_WG) void k_scan_chunks(const RunDesc* __restrict__ runs,
                                                         RunState* __restrict__ st,
                                                         int* __restrict__ chunk_sums,
-                                                        int* __restrict__ offsets) {
+                                                        int* __restrict__ offsets,
+                                                        int* __restrict__ toff,
+                                                        int* __restrict__ big_count) {
   __shared__ int wsum[4];
   const RunDesc rd = runs[blockIdx.x];
   const int total = block_exclusive_scan(chunk_sums + rd.cell_item_begin,
@@ -174,20 +186,30 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_chunks(const RunDesc* __restric
   if (threadIdx.x == 0) {
     offsets[rd.cbase + rd.nx * rd.ny] = total;
     st[blockIdx.x].nsorted = total;
+    if (toff) toff[rd.toffbase + rd.nbuckets] = total;
+    if (blockIdx.x == 0) *big_count = 0;
   }
 }
 
+// the chunk's base added to its offsets; with toff (a tile-major cell numbering: tile t owns
+// cells [t T^2, (t + 1) T^2)) also the tile offsets toff[t] = offsets[t T^2]
 __global__ __launch_bounds__(PIC_WG) void k_scan_add(const RunDesc* __restrict__ runs,
                                                      int* __restrict__ offsets,
                                                      const int* __restrict__ chunk_sums,
-                                                     const int2* __restrict__ items) {
+                                                     const int2* __restrict__ items,
+                                                     int* __restrict__ toff) {
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
   const int G = rd.nx * rd.ny;
+  const int TT = rd.tile * rd.tile;
   const int add = chunk_sums[blockIdx.x];
   for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
     int c = it.y + k;
-    if (c < G) offsets[rd.cbase + c] += add;
+    if (c < G) {
+      const int o = offsets[rd.cbase + c] + add;
+      offsets[rd.cbase + c] = o;
+      if (toff && c % TT == 0) toff[rd.toffbase + c / TT] = o;
+    }
   }
 }
 
@@ -686,18 +708,6 @@ __global__ __launch_bounds__(64) void k_tile_scatter(const RunDesc* __restrict__
   }
 }
 
-// tile offsets of a tile-major cell sort: toff[t] = offsets[t * T * T]
-__global__ __launch_bounds__(PIC_WG) void k_toff_from_cells(const RunDesc* __restrict__ runs,
-                                                            const int* __restrict__ offsets,
-                                                            int* __restrict__ toff, int nruns) {
-  for (int r = 0; r < nruns; ++r) {
-    const RunDesc rd = runs[r];
-    const int TT = rd.tile * rd.tile;
-    for (int t = blockIdx.x * PIC_WG + threadIdx.x; t <= rd.nbuckets; t += gridDim.x * PIC_WG)
-      toff[rd.toffbase + t] = offsets[rd.cbase + (long long)t * TT];
-  }
-}
-
 __global__ void k_fill_i32(int* p, long long n, int v) {
   for (long long i = blockIdx.x * (long long)PIC_WG + threadIdx.x; i < n;
        i += (long long)gridDim.x * PIC_WG)
@@ -990,36 +1000,38 @@ void launch_reduce_sum(hipStream_t s, const double* parts, int n, double* out) {
 
 static void sort_tail(hipStream_t s, const RunDesc* runs, RunState* st, const int* id,
                       SortBufs sb, const int2* citems, int ncitems) {
-  launch_fill_i32(s, sb.big_count, 1, 0);
   k_segsort_small<<<sb.nsegitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list,
                                                   sb.big_count, sb.segitems);
   k_segsort_big<<<256, PIC_WG, 0, s>>>(runs, sb.offsets, sb.perm, id, sb.big_list, sb.big_count,
                                        sb.scratch);
 }
 
+// clear: zero the counts once scanned; toff: also write the tile offsets (nullptr: not)
 static void sort_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, SortBufs sb,
-                      const int2* citems, int ncitems) {
-  k_scan_local<<<ncitems, PIC_WG, 0, s>>>(runs, sb.counts, sb.offsets, sb.chunk_sums, citems);
-  k_scan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.chunk_sums, sb.offsets);
-  k_scan_add<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.chunk_sums, citems);
+                      const int2* citems, int ncitems, int clear, int* toff) {
+  k_scan_local<<<ncitems, PIC_WG, 0, s>>>(runs, sb.counts, sb.offsets, sb.chunk_sums, citems,
+                                          clear);
+  k_scan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.chunk_sums, sb.offsets, toff,
+                                         sb.big_count);
+  k_scan_add<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.chunk_sums, citems, toff);
 }
 
 template <typename R>
 void launch_sort(hipStream_t s, const RunDesc* runs, RunState* st, int nruns, PartBufs<R> src,
                  PartBufs<R> dst, SortBufs sb, const int2* pitems, int npitems,
                  const int2* citems, int ncitems, long long ctotal, bool move_data) {
-  launch_fill_i32(s, sb.counts, ctotal, 0);
+  // sb.counts is zero here: zeroed at allocation and by the scan of every sort since
+  (void)ctotal;
   k_sort_rank<R, true><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, src.x, src.y,
                                                                    nullptr, sb.keys,
                                                  sb.rank, sb.counts, pitems);
-  sort_scan(s, runs, st, nruns, sb, citems, ncitems);
+  sort_scan(s, runs, st, nruns, sb, citems, ncitems, 1, sb.toff);
   k_sort_scatter<<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, sb.keys, sb.rank,
                                                               sb.offsets, sb.perm, pitems);
   sort_tail(s, runs, st, src.id, sb, citems, ncitems);
   if (move_data)
     k_sort_apply<R><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, sb.perm, src, dst,
                                                                  pitems);
-  k_toff_from_cells<<<64, PIC_WG, 0, s>>>(runs, sb.offsets, sb.toff, nruns);
 }
 
 template <typename R>
@@ -1053,7 +1065,7 @@ void launch_sort_keys_only(hipStream_t s, const RunDesc* runs, RunState* st, con
   k_sort_rank<float, false><<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, nullptr,
                                                                         nullptr, keys,
                                                       sb.keys, sb.rank, sb.counts, pitems);
-  sort_scan(s, runs, st, 1, sb, citems, ncitems);
+  sort_scan(s, runs, st, 1, sb, citems, ncitems, 0, nullptr);
   k_sort_scatter<<<npitems * PIC_SORT_SLICES, PIC_WG, 0, s>>>(runs, st, keys, sb.rank, sb.offsets,
                                                               sb.perm, pitems);
   sort_tail(s, runs, st, nullptr, sb, citems, ncitems);
