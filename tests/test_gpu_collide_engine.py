@@ -115,7 +115,7 @@ def engine_run(cfg, parts, nsteps, graphs=True, tiling=None):
 @pytest.mark.parametrize("graphs", [True, False])
 def test_engine_dsmc_matches_oracle_steps(oracle, graphs):
     cfg, parts = blob(coeff=0.01)
-    nsteps = 8
+    nsteps = 11  # with graphs: one captured 8-step sequence, then 3 eager steps
     print(f"engine dsmc vs oracle, graphs={graphs}", flush=True)
     got, gcnt = engine_run(cfg, parts, nsteps, graphs)
     want, wcnt = oracle_run(oracle, cfg, parts, nsteps)
