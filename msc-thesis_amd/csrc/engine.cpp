@@ -89,7 +89,55 @@ static Tuning default_tuning(const pic_config& c, int tile, int halo, int splits
   // about 8192 particles per 512-thread workgroup (1 to 64 workgroups per tile)
   const long long per_tile = std::max<long long>(c.capacity, 1) / ntiles;
   t.splits = splits ? splits : (int)std::max(1LL, std::min(64LL, (per_tile + 8191) / 8192));
+  t.per_tile = splits == 0 && c.collide == PIC_COLLIDE_DSMC;
   return t;
+}
+
+// Items (run, tile x, tile y, split | splits << 16) of run r: t.splits per tile, or with
+// per-tile splits (and the tile offsets of the current order on the host) one split per
+// PUSH_SPLIT_TARGET particles of the tile, 1 to 64.
+void Ctx::push_items_for(int r, const int* toff_h) {
+  RunDesc& d = rd[r];
+  const Tuning& t = tune[r];
+  constexpr long long PUSH_SPLIT_TARGET = 4096;
+  d.push_item_begin = (int)push_items.size();
+  for (int tx = 0; tx < d.tiles_x; ++tx)
+    for (int ty = 0; ty < d.tiles_y; ++ty) {
+      int S = t.splits;
+      if (t.per_tile && toff_h) {
+        const long long tb = d.toffbase + (long long)tx * d.tiles_y + ty;
+        const long long n = toff_h[tb + 1] - toff_h[tb];
+        S = (int)std::max(1LL, std::min(64LL, (n + PUSH_SPLIT_TARGET - 1) / PUSH_SPLIT_TARGET));
+      }
+      for (int s = 0; s < S; ++s) push_items.push_back(make_int4(r, tx, ty, s | (S << 16)));
+    }
+  if (cfg[r].collide == PIC_COLLIDE_MCC) {
+    const long long preg = round_up(std::max<long long>(d.pcap, 1), PIC_CHUNK);
+    for (long long s = 0; s < preg; s += PIC_CHUNK) push_items.push_back(make_int4(r, -1, (int)s, 0));
+  }
+  d.push_item_end = (int)push_items.size();
+}
+
+// Per-tile splits follow the tile counts of the current order: rebuilt from them (after a
+// sort) when a run asks for them.  Only the push items, their partials and the captured
+// sequences change.
+int Ctx::rebalance_push_items() {
+  bool any = false;
+  for (int r = 0; r < nruns; ++r) any = any || (tune[r].per_tile && has_particles[r]);
+  if (!any) return PIC_OK;
+  PIC_HIP(hipStreamSynchronize(stream));
+  std::vector<int> toff_h((size_t)toffsum);
+  PIC_HIP(hipMemcpy(toff_h.data(), toffs[cur], sizeof(int) * toffsum, hipMemcpyDeviceToHost));
+  push_items.clear();
+  for (int r = 0; r < nruns; ++r) push_items_for(r, has_particles[r] ? toff_h.data() : nullptr);
+  if (d_push) (void)hipFree(d_push);
+  PIC_HIP(dalloc(&d_push, push_items.size()));
+  PIC_HIP(hipMemcpy(d_push, push_items.data(), sizeof(int4) * push_items.size(),
+                    hipMemcpyHostToDevice));
+  if (partials_push) (void)hipFree(partials_push);
+  PIC_HIP(dalloc(&partials_push, push_items.size() * 3));
+  free_graphs();
+  return upload_runs();
 }
 
 Ctx::~Ctx() {
@@ -128,7 +176,8 @@ int Ctx::build_items() {
   pitems.clear(); citems.clear(); rowpair.clear(); cols.clear(); rows.clear(); push_items.clear();
   segitems.clear(); tscan_items.clear();
   lds_push = 0;
-  long long tsum = 0, toffsum = 0;
+  long long tsum = 0;
+  toffsum = 0;
   int maxb = 1;
   long long rows_total = 0;  // rows of all runs: sets the FFT row grouping
   for (int r = 0; r < nruns; ++r) rows_total += cfg[r].nx;
@@ -161,13 +210,7 @@ int Ctx::build_items() {
     for (int c = 0; c < d.nx * d.ny; c += PIC_CELL_CHUNK) citems.push_back(make_int2(r, c));
     d.cell_item_end = (int)citems.size();
     for (int c = 0; c < d.nx * d.ny; c += PIC_SEG_CELLS) segitems.push_back(make_int2(r, c));
-    d.push_item_begin = (int)push_items.size();
-    for (int tx = 0; tx < d.tiles_x; ++tx)
-      for (int ty = 0; ty < d.tiles_y; ++ty)
-        for (int s = 0; s < t.splits; ++s) push_items.push_back(make_int4(r, tx, ty, s));
-    if (cfg[r].collide == PIC_COLLIDE_MCC)
-      for (long long s = 0; s < preg; s += PIC_CHUNK) push_items.push_back(make_int4(r, -1, (int)s, 0));
-    d.push_item_end = (int)push_items.size();
+    push_items_for(r, nullptr);
     d.fchunks = 0;
     fft_groups(rows_total, d.nx, d.ny, &d.fft_pp, &d.fft_rf);
     for (int i = 0; i < d.nx; i += 2 * d.fft_pp) rowpair.push_back(make_int2(r, i));
@@ -768,6 +811,8 @@ int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void* py_,
   PIC_HIP(hipMemsetAsync(x.d_coll_counts + 4 * run, 0, sizeof(long long) * 4, x.stream));
   x.launch_sort_stage();
   x.steps_since_sort = 0;
+  x.has_particles[run] = 1;
+  if ((rc = x.rebalance_push_items())) return rc;
   // deposit the initial charge of this run only
   if (x.fp64) {
     PushArgs<double> a = x.push_args<double>(1);
@@ -1009,7 +1054,7 @@ int pic_get_tuning(pic_ctx* ctx, int* tile, int* halo, int* splits, int* sort_ev
   if (x.tune.empty()) return fail(PIC_ERR_STATE, "context not allocated yet");
   if (tile) *tile = x.tune[0].tile;
   if (halo) *halo = x.tune[0].halo;
-  if (splits) *splits = x.tune[0].splits;
+  if (splits) *splits = x.tune[0].per_tile ? 0 : x.tune[0].splits;
   if (sort_every) *sort_every = x.sort_every;
   return PIC_OK;
 }
@@ -1045,7 +1090,7 @@ int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every)
     x.steps_since_sort = 0;
     PIC_HIP(hipStreamSynchronize(x.stream));
   }
-  return PIC_OK;
+  return x.rebalance_push_items();
 }
 
 // Project autotuner: times (tile, halo, sort cadence) candidates on the live state and
@@ -1126,6 +1171,7 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
         if ((rc = x.build_items())) break;
         x.launch_sort_stage();
         x.steps_since_sort = 0;
+        if ((rc = x.rebalance_push_items())) break;
         // one warm cycle (captures graphs), then the timed cycles
         const int ml = x.macro_len();
         if ((rc = x.step(ml))) break;

@@ -30,6 +30,9 @@ double mcc_nu_max(const pic_config& c);
 
 struct Tuning {
   int tile, halo, splits;
+  // splits per tile from each tile's particle count (DSMC runs asked for splits 0: the
+  // blob-like densities of the collision workloads leave uniform splits unbalanced)
+  bool per_tile = false;
 };
 
 struct Ctx {
@@ -67,6 +70,7 @@ struct Ctx {
   double2* tw = nullptr;
   SortBufs sb{};
   int* toffs[2] = {nullptr, nullptr};  // tile offsets of the order in buffer cur / 1 - cur
+  long long toffsum = 0;               // entries of each toffs[] array
   bool fused = false;                  // sort fused into the push (see build_items)
   std::vector<int2> pitems, citems, rowpair, cols, rows, segitems, tscan_items, fscan_items;
   std::vector<int4> push_items;
@@ -98,6 +102,8 @@ struct Ctx {
   ~Ctx();
   int alloc();
   int build_items();
+  void push_items_for(int r, const int* toff_h);
+  int rebalance_push_items();
   int upload_runs();
   void free_graphs();
 
