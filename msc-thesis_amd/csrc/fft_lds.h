@@ -14,11 +14,20 @@ __device__ __forceinline__ void fft_bfly(double2& u, double2& v, double2 w) {
   v = make_double2(a.x - tr, a.y - ti);
 }
 
+// LDS slot of element i of a padded buffer (PAD = 1: one spare slot after every 8 values,
+// so that the 8 values a thread of the first pass takes are not 128 B apart from the next
+// thread's: a 16-byte load of 64 lanes at that stride hits the same banks).  Rows of n >= 8
+// values stay back to back: row r starts at slot r (n + n/8).
+template <int PAD>
+__device__ __forceinline__ int fft_slot(int i) {
+  return PAD ? i + (i >> 3) : i;
+}
+
 // Stages s .. s+K-1 of the radix-2 transform below, in registers: a thread loads the 2^K
 // elements i0 + m h (h = 2^(s-1)) that those stages combine, applies the same butterflies
 // with the same twiddles in the same order as stage-by-stage radix-2 (so the result is
 // bit-identical to it), and stores them back.
-template <int K>
+template <int K, int PAD>
 __device__ __forceinline__ void fft_lds_pass(double2* a, int rows, int n, int logn, int s,
                                              double ws, const double2* __restrict__ tw,
                                              int logtw) {
@@ -28,10 +37,10 @@ __device__ __forceinline__ void fft_lds_pass(double2* a, int rows, int n, int lo
   for (int b = threadIdx.x; b < total; b += PIC_WG) {
     const int r = b >> (logn - K), g = b & gmask;
     const int j = g & (h - 1);
-    double2* row = a + r * n + ((g >> (s - 1)) << (s - 1 + K)) + j;
+    const int e0 = r * n + ((g >> (s - 1)) << (s - 1 + K)) + j;
     double2 v[1 << K];
 #pragma unroll
-    for (int m = 0; m < (1 << K); ++m) v[m] = row[m * h];
+    for (int m = 0; m < (1 << K); ++m) v[m] = a[fft_slot<PAD>(e0 + m * h)];
 #pragma unroll
     for (int t = 0; t < K; ++t) {
       const int tsh = logtw - s - t;
@@ -44,7 +53,7 @@ __device__ __forceinline__ void fft_lds_pass(double2* a, int rows, int n, int lo
       }
     }
 #pragma unroll
-    for (int m = 0; m < (1 << K); ++m) row[m * h] = v[m];
+    for (int m = 0; m < (1 << K); ++m) a[fft_slot<PAD>(e0 + m * h)] = v[m];
   }
 }
 
@@ -54,21 +63,22 @@ __device__ __forceinline__ void fft_lds_pass(double2* a, int rows, int n, int lo
 // global table (logtw = PIC_FFT_LOG) or a copy of the n-point twiddles in LDS
 // (fft_stage_twiddles, logtw = logn).  Radix-2 butterflies, three stages per pass in
 // registers (one barrier per pass), the groups of all rows spread over the workgroup.  All
-// threads of the workgroup (PIC_WG) must call it.
+// threads of the workgroup (PIC_WG) must call it.  PAD: the buffer's layout (fft_slot).
+template <int PAD = 0>
 __device__ __forceinline__ void fft_lds_rows(double2* a, int rows, int n, int logn, int sign,
                                              const double2* __restrict__ tw,
                                              int logtw = PIC_FFT_LOG) {
   const double ws = (sign < 0) ? 1.0 : -1.0;
   int s = 1;
   for (; s + 2 <= logn; s += 3) {
-    fft_lds_pass<3>(a, rows, n, logn, s, ws, tw, logtw);
+    fft_lds_pass<3, PAD>(a, rows, n, logn, s, ws, tw, logtw);
     __syncthreads();
   }
   if (s + 1 == logn) {
-    fft_lds_pass<2>(a, rows, n, logn, s, ws, tw, logtw);
+    fft_lds_pass<2, PAD>(a, rows, n, logn, s, ws, tw, logtw);
     __syncthreads();
   } else if (s == logn) {
-    fft_lds_pass<1>(a, rows, n, logn, s, ws, tw, logtw);
+    fft_lds_pass<1, PAD>(a, rows, n, logn, s, ws, tw, logtw);
     __syncthreads();
   }
 }

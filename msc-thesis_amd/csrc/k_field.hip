@@ -98,26 +98,27 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_rows(FieldArgs a) {
   }
 }
 
-__global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
-  extern __shared__ double2 lds[];
-  if (blockIdx.x >= a.n_col_items) {  // extra workgroups: the count-table scan, one per run
-    fscan_base(a.runs, a.st, a.fchunk_sums, a.fstride, a.ftoff, blockIdx.x - a.n_col_items);
-    return;
-  }
+// One column: forward FFT, Green 1/k^2, derivative symbols, inverse FFTs.  The LDS rows are
+// padded when PAD (fft_slot): a 16-byte load of 64 lanes 128 B apart, the first pass's
+// pattern, hits the same banks.
+template <int PAD>
+__device__ __forceinline__ void fft_col_item(const FieldArgs& a, double2* lds) {
   const int2 it = a.col_items[blockIdx.x];
   const RunDesc rd = a.runs[it.x];
   const int nx = rd.nx, ny = rd.ny, lognx = rd.lognx;
   const int k = it.y;
+  // nxp slots per row; A, B, C back to back (row r of a buffer starts at slot r nxp)
+  const int nxp = fft_slot<PAD>(nx), mxp = a.max_nx + (a.max_nx >> 3);
   double2* col = lds;
-  double2* A = lds + nx;
-  double2* B = lds + 2 * nx;
-  double2* C = lds + 3 * nx;
-  double2* twl = lds + (a.want_phi ? 4 : 3) * a.max_nx;
+  double2* A = lds + nxp;
+  double2* B = lds + 2 * nxp;
+  double2* C = lds + 3 * nxp;
+  double2* twl = lds + (a.want_phi ? 4 : 3) * mxp;
   double2* S = a.S + rd.sbase + (long long)k * nx;
   fft_stage_twiddles(twl, nx, a.tw);
-  for (int i = threadIdx.x; i < nx; i += PIC_WG) col[bitrev(i, lognx)] = S[i];
+  for (int i = threadIdx.x; i < nx; i += PIC_WG) col[fft_slot<PAD>(bitrev(i, lognx))] = S[i];
   __syncthreads();
-  fft_lds_rows(col, 1, nx, lognx, -1, twl, lognx);
+  fft_lds_rows<PAD>(col, 1, nx, lognx, -1, twl, lognx);
   if (k == 0 && threadIdx.x == 0 && !a.rho_in) {
     // the one reader and writer of the run's step counter in this kernel
     const long long stp = a.st[it.x].step;
@@ -139,24 +140,36 @@ __global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
     const int mm = (2 * m < nx) ? m : m - nx;
     const double kx = twopi * mm / rd.lx;
     const double k2 = kx * kx + ky * ky;
-    const double2 r = col[m];
+    const double2 r = col[fft_slot<PAD>(m)];
     double2 ph = make_double2(0.0, 0.0);
     if (m != 0 || k != 0) ph = make_double2(r.x / k2, r.y / k2);
     const double sx = sin2pi_frac(m, nx, twl, lognx) / rd.dx;
-    const int br = bitrev(m, lognx);
+    const int br = fft_slot<PAD>(bitrev(m, lognx));
     A[br] = make_double2(sx * ph.y, -sx * ph.x);
     B[br] = make_double2(sy * ph.y, -sy * ph.x);
     if (a.want_phi) C[br] = ph;
   }
   __syncthreads();
-  fft_lds_rows(A, a.want_phi ? 3 : 2, nx, lognx, +1, twl, lognx);  // A, B, C consecutive
+  fft_lds_rows<PAD>(A, a.want_phi ? 3 : 2, nx, lognx, +1, twl, lognx);  // A, B, C consecutive
   double2* Sx = a.Sx + rd.sbase + (long long)k * nx;
   double2* Sy = a.Sy + rd.sbase + (long long)k * nx;
   for (int i = threadIdx.x; i < nx; i += PIC_WG) {
-    Sx[i] = A[i];
-    Sy[i] = B[i];
-    if (a.want_phi) S[i] = C[i];
+    const int q = fft_slot<PAD>(i);
+    Sx[i] = A[q];
+    Sy[i] = B[q];
+    if (a.want_phi) S[i] = C[q];
   }
+}
+
+__global__ __launch_bounds__(PIC_WG) void k_fft_cols(FieldArgs a) {
+  extern __shared__ double2 lds[];
+  if (blockIdx.x >= a.n_col_items) {  // extra workgroups: the count-table scan, one per run
+    fscan_base(a.runs, a.st, a.fchunk_sums, a.fstride, a.ftoff, blockIdx.x - a.n_col_items);
+    return;
+  }
+  // padded rows need whole groups of 8 values (every grid is a power of two >= 4)
+  if (a.runs[a.col_items[blockIdx.x].x].nx >= 8) fft_col_item<1>(a, lds);
+  else fft_col_item<0>(a, lds);
 }
 
 // MODE 0: row i of (Ex + i Ey) -> padded E + field energy.  MODE 1: rows (i, i+1) of phi.
@@ -293,7 +306,8 @@ inline int grid_for(long long n) {
 void launch_field(hipStream_t s, const FieldArgs& a, hipEvent_t* ev) {
   // data, then the n/2 twiddles of the transform length
   const size_t lrow = ((size_t)a.row_lds + a.max_ny / 2) * sizeof(double2);
-  const size_t lcol = (size_t)a.max_nx * sizeof(double2) * (a.want_phi ? 4 : 3) +
+  const size_t lcol = (size_t)(a.max_nx + a.max_nx / 8) * sizeof(double2) *
+                          (a.want_phi ? 4 : 3) +
                       (size_t)(a.max_nx / 2) * sizeof(double2);
   // extra workgroups of the folded end-of-step work
   const int xrows = (a.fold_finish ? a.nruns : 0) + (a.fold_scan ? a.nfscan_items : 0);
