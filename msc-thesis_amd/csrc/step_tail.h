@@ -56,10 +56,28 @@ __device__ __forceinline__ void step_finish_run(const RunDesc* __restrict__ runs
   }
   const RunDesc rd = runs[run];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = rd.push_item_begin + threadIdx.x; i < rd.push_item_end; i += PIC_WG) {
-    v[0] += ppush[3 * (long long)i + 0];
-    v[1] += ppush[3 * (long long)i + 1];
-    v[2] += ppush[3 * (long long)i + 2];
+  // each thread adds its items i, i + PIC_WG, ... in that order; FINISH_U of them are loaded
+  // together (past the end: the last item, added as zero), so a run with many push items
+  // (the collision runs' tail chunks) waits for one load latency per FINISH_U items
+  constexpr int FINISH_U = 8;
+  const int pe = rd.push_item_end;
+  for (int i0 = rd.push_item_begin + threadIdx.x; i0 < pe; i0 += FINISH_U * PIC_WG) {
+    double a0[FINISH_U], a1[FINISH_U], a2[FINISH_U];
+#pragma unroll
+    for (int u = 0; u < FINISH_U; ++u) {
+      const int i = i0 + u * PIC_WG;
+      const long long ic = 3 * (long long)(i < pe ? i : pe - 1);
+      a0[u] = ppush[ic + 0];
+      a1[u] = ppush[ic + 1];
+      a2[u] = ppush[ic + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < FINISH_U; ++u) {
+      const bool ok = i0 + u * PIC_WG < pe;
+      v[0] += ok ? a0[u] : 0.0;
+      v[1] += ok ? a1[u] : 0.0;
+      v[2] += ok ? a2[u] : 0.0;
+    }
   }
   for (int q = rd.row_item_begin + threadIdx.x; q < rd.row_item_end; q += PIC_WG) v[3] += prows[q];
   // the four workgroup sums in one pass (each: wave sums, then the four waves in order)
