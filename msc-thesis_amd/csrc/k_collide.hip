@@ -153,9 +153,26 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   __shared__ unsigned long long s[2048];
   __shared__ unsigned long long red[4][4];
   {
+    // the k_mcc workgroups' counts, eight per thread loaded together (integer sums: any
+    // order gives the same totals)
     unsigned long long c[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += PIC_WG)
-      for (int k = 0; k < 4; ++k) c[k] += (unsigned long long)a.coll_partials[4 * b + k];
+    constexpr int BU = 8;
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += BU * PIC_WG) {
+      int4 v[BU];
+#pragma unroll
+      for (int u = 0; u < BU; ++u) {
+        const int b = b0 + u * PIC_WG;
+        v[u] = reinterpret_cast<const int4*>(a.coll_partials)[b < nblocks ? b : nblocks - 1];
+      }
+#pragma unroll
+      for (int u = 0; u < BU; ++u)
+        if (b0 + u * PIC_WG < nblocks) {
+          c[0] += (unsigned long long)v[u].x;
+          c[1] += (unsigned long long)v[u].y;
+          c[2] += (unsigned long long)v[u].z;
+          c[3] += (unsigned long long)v[u].w;
+        }
+    }
     for (int k = 0; k < 4; ++k)
       for (int o = 32; o > 0; o >>= 1) c[k] += __shfl_xor(c[k], o, 64);
     if ((threadIdx.x & 63) == 0)
