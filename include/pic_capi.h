@@ -83,7 +83,8 @@ typedef struct pic_config {
   int32_t sort_every;        /* steps between cell sorts; 0 => autotuner default */
   int32_t tile;              /* deposit tile side in cells (power of two); 0 => default */
   int32_t halo;              /* deposit LDS window halo in cells; 0 => default */
-  int32_t splits;            /* workgroups per tile; 0 => default */
+  int32_t splits;            /* workgroups per tile; 0 => default (sized by the particles
+                                per tile; in DSMC runs per tile, from each tile's count) */
   int32_t collide;           /* PIC_COLLIDE_* */
   int32_t reserved0;
   uint64_t seed;             /* Philox4x32-10 key for every random stream */
@@ -151,7 +152,8 @@ int pic_current_step(pic_ctx* ctx, int run, int64_t* step);
 int pic_stream(pic_ctx* ctx, void** stream);  /* hipStream_t of the context */
 /* Use (1) or bypass (0) captured hipGraphs for the step loop (default 1). */
 int pic_use_graphs(pic_ctx* ctx, int enable);
-/* Tiling / sort cadence in use (after defaults or autotuning). */
+/* Tiling / sort cadence in use (after defaults or autotuning); splits 0 = the default
+ * sizing (per tile in DSMC runs). */
 int pic_get_tuning(pic_ctx* ctx, int* tile, int* halo, int* splits, int* sort_every);
 int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every);
 /* 1 when the tile sort runs inside the push launches (the step before a sort step counts
