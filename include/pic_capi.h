@@ -152,8 +152,8 @@ int pic_current_step(pic_ctx* ctx, int run, int64_t* step);
 int pic_stream(pic_ctx* ctx, void** stream);  /* hipStream_t of the context */
 /* Use (1) or bypass (0) captured hipGraphs for the step loop (default 1). */
 int pic_use_graphs(pic_ctx* ctx, int enable);
-/* Tiling / sort cadence in use (after defaults or autotuning); splits 0 = the default
- * sizing (per tile in DSMC runs). */
+/* Tiling / sort cadence in use (after defaults or autotuning); splits reads 0 when the
+ * splits are sized per tile (DSMC runs asked for the default). */
 int pic_get_tuning(pic_ctx* ctx, int* tile, int* halo, int* splits, int* sort_every);
 int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every);
 /* 1 when the tile sort runs inside the push launches (the step before a sort step counts
