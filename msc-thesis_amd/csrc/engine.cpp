@@ -112,8 +112,11 @@ void Ctx::push_items_for(int r, const int* toff_h) {
       for (int s = 0; s < S; ++s) push_items.push_back(make_int4(r, tx, ty, s | (S << 16)));
     }
   if (cfg[r].collide == PIC_COLLIDE_MCC) {
-    const long long preg = round_up(std::max<long long>(d.pcap, 1), PIC_CHUNK);
-    for (long long s = 0; s < preg; s += PIC_CHUNK) push_items.push_back(make_int4(r, -1, (int)s, 0));
+    // tail items: the particles appended since the last sort (births), PIC_TAIL_CHUNK
+    // indices each
+    const long long preg = round_up(std::max<long long>(d.pcap, 1), PIC_TAIL_CHUNK);
+    for (long long s = 0; s < preg; s += PIC_TAIL_CHUNK)
+      push_items.push_back(make_int4(r, -1, (int)s, 0));
   }
   d.push_item_end = (int)push_items.size();
 }
@@ -220,11 +223,13 @@ int Ctx::build_items() {
     d.row_item_end = (int)rows.size();
   }
   if (lds_push > 160 * 1024) return fail(PIC_ERR_INVALID_CONFIG, "deposit window exceeds LDS");
-  // Sort fused into the push: needs no births or cell lists (no collisions), a cadence of
-  // at least two steps (the count is taken one step ahead), and room in LDS for one
-  // counter per tile (int) and two per tile and wave (16-bit pass counts).
+  // Sort fused into the push: needs no cell lists (no DSMC), a cadence of at least two
+  // steps (the count is taken one step ahead), and room in LDS for one counter per tile
+  // (int) and two per tile and wave (16-bit pass counts).  MCC births join it through the
+  // tail items and the births kernel (the count step's births are counted as they are
+  // appended).
   const int fused_lds = (PUSH_NW + 1) * maxb * (int)sizeof(int);
-  fused = !any_mcc && !any_dsmc && sort_every >= 2 && maxb <= 512 &&
+  fused = !any_dsmc && sort_every >= 2 && maxb <= 512 &&
           lds_push + fused_lds <= 160 * 1024;
   fscan_items.clear();
   long long fsum = 0;
@@ -508,6 +513,7 @@ int Ctx::launch_collide_stage() {
       a.counts = d_coll_counts + 4 * r;
       a.birth_count = birth_count; a.birth_parent = birth_parent; a.birth_buf = birth_buf;
       a.coll_list = coll_list; a.coll_count = coll_count; a.coll_partials = coll_partials;
+      a.count_table = collide_count_table;
       a.from_state_step = 1;
       a.diag = diag + d.dbase * 8;
       launch_mcc(stream, a, d.pcap);
@@ -563,24 +569,33 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
   if (ev) (void)hipEventRecord(ev[7], stream);  // the push stage is the k_push launch alone
   if (defer_tail) fin_pending = true;
   else launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
+  // with MCC the count table is scanned after the collision stage, whose births add their
+  // tiles to it
+  const bool scan_after_collide = any_mcc;
   if (smode == 1) {
     sb.toff = toffs[1 - cur];
     if (defer_tail) {
       scan_pending = true;
       scan_toff = toffs[1 - cur];
-    } else {
+    } else if (!scan_after_collide) {
       launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
     }
   }
+  // the sort step's push wrote the new order into the other buffer: the collision stage
+  // works on that one
+  if (smode == 2) cur ^= 1;
   if (ev) (void)hipEventRecord(ev[4], stream);
+  collide_count_table = (smode == 1) ? sb.ftable : nullptr;
   launch_collide_stage();
+  collide_count_table = nullptr;
+  if (smode == 1 && !defer_tail && scan_after_collide)
+    launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
   if (ev) (void)hipEventRecord(ev[5], stream);
   if (any_dsmc) {
     steps_since_sort = 0;
   } else if (allow_sort) {
     ++steps_since_sort;
     if (smode == 2) {
-      cur ^= 1;
       steps_since_sort = 0;
     } else if (!fused && steps_since_sort >= sort_every) {
       launch_sort_stage();

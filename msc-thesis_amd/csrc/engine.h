@@ -71,6 +71,8 @@ struct Ctx {
   SortBufs sb{};
   int* toffs[2] = {nullptr, nullptr};  // tile offsets of the order in buffer cur / 1 - cur
   long long toffsum = 0;               // entries of each toffs[] array
+  int* collide_count_table = nullptr;  // the count table while a count step's collision
+                                       // stage runs (its births are counted; else NULL)
   bool fused = false;                  // sort fused into the push (see build_items)
   std::vector<int2> pitems, citems, rowpair, cols, rows, segitems, tscan_items, fscan_items;
   std::vector<int4> push_items;

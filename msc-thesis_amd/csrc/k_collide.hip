@@ -238,6 +238,18 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
     atomicAdd(&rh[(long long)cxp * rd.ny + cy], (unsigned long long)(ix1 * iy0));
     atomicAdd(&rh[(long long)cx * rd.ny + cyp], (unsigned long long)(ix0 * iy1));
     atomicAdd(&rh[(long long)cxp * rd.ny + cyp], (unsigned long long)(ix1 * iy1));
+    if (a.count_table) {
+      // this step counted the tiles for the sort that the next step fuses into its push:
+      // the birth (index np + t, pushed by tail item (np + t) / PIC_TAIL_CHUNK, which follows
+      // the run's tile items) is counted there, by the tile of its position (collision runs
+      // key on positions)
+      const int sh = __builtin_ctz((unsigned)rd.tile);
+      const int key = (cx >> sh) * rd.tiles_y + (cy >> sh);
+      const int ntail = (rd.pcap + PIC_TAIL_CHUNK - 1) / PIC_TAIL_CHUNK;
+      const long long row =
+          rd.ftbase + (long long)(rd.fchunks - ntail + (np + t) / PIC_TAIL_CHUNK) * rd.nbuckets;
+      atomicAdd(&a.count_table[row + key], 1);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -239,6 +239,9 @@ struct MccArgs {
                                   // per_block entries per workgroup (mcc_list_layout)
   int* coll_count;                // [blocks] entries of each region
   int* coll_partials;             // [blocks][4] collision counts of each k_mcc workgroup
+  int* count_table;               // sort fused into the push, after its count step: the
+                                  // count table, to which the births add their tiles (in
+                                  // the row of the tail item that will push them); else NULL
 };
 // MCC collision list layout for n_upper particles: the null-test grid and the list region
 // of each workgroup (its grid-stride share); the list needs blocks * per_block entries

@@ -18,6 +18,7 @@
 #define PIC_FFT_LOG 11               // log2(PIC_FFT_MAX)
 #define PIC_TW_LOG 11                // twiddle table: exp(-2 pi i k / 2048), k < 1024
 #define PIC_CHUNK 4096               // particles per chunk item
+#define PIC_TAIL_CHUNK 65536         // particle range of a push tail item (births)
 #define PIC_DIAG_CAP 65536           // diagnostics ring depth (steps)
 
 struct RunDesc {

@@ -147,7 +147,12 @@ def test_sort_runs_fused_only_where_it_can():
     _lib.check(_lib.lib.pic_mcc_preset(ctypes.byref(cfg), 1e21, 2.0, 1e15))
     parts = pic_amd.load_landau(64, 64, 64.0, 64.0, 4, vth=2.0, alpha=0.0, mode=0, seed=5)
     cfg.capacity = int(parts[0].size * 1.25)
-    with Simulation(cfg) as sim:                            # births need the separate sort
+    with Simulation(cfg) as sim:            # MCC: births join the fused sort as tail items
+        sim.set_particles(*parts)
+        assert sim.sort_fused()
+    cfg.collide = _lib.PIC_COLLIDE_DSMC
+    cfg.dsmc_coeff = 1e-3
+    with Simulation(cfg) as sim:            # DSMC: cell lists every step, the separate sort
         sim.set_particles(*parts)
         assert not sim.sort_fused()
 
