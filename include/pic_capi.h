@@ -157,8 +157,9 @@ int pic_use_graphs(pic_ctx* ctx, int enable);
 int pic_get_tuning(pic_ctx* ctx, int* tile, int* halo, int* splits, int* sort_every);
 int pic_set_tuning(pic_ctx* ctx, int tile, int halo, int splits, int sort_every);
 /* 1 when the tile sort runs inside the push launches (the step before a sort step counts
- * the tiles of the new positions, the sort step scatters while it pushes); 0 when it runs
- * as separate kernels (collisions on, sort_every < 2, or too many tiles for the LDS). */
+ * the tiles of the new positions, the sort step scatters while it pushes; MCC births
+ * included); 0 when it runs as separate kernels (DSMC runs, sort_every < 2, or too many
+ * tiles for the LDS). */
 int pic_sort_fused(pic_ctx* ctx, int* fused);
 /* Time candidate tilings for `trial_steps` each and keep the fastest (restores state). */
 int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms_per_step);
