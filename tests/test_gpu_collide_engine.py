@@ -158,3 +158,26 @@ def test_engine_mcc_matches_oracle_steps(oracle, graphs):
     assert wcnt[2] > 0 and got[0].size == parts[0].size + wcnt[2]
     for g, w in zip(got, want):
         assert np.array_equal(g, w)
+
+
+def test_engine_mcc_births_through_the_fused_sort():
+    """MCC births joining the sort fused into the push (sort every 4: the tail items count
+    and scatter the particles appended since the last sort, the count step's own births are
+    counted by the births kernel) give the same particles, ids and counts as the separate
+    sort every step."""
+    cfg, parts = mcc_case()
+    nsteps = 21
+    out = []
+    for se in (4, 1):
+        with Simulation(cfg) as sim:
+            sim.set_tuning(16, 8, 0, se)
+            sim.set_particles(*parts)
+            assert sim.sort_fused() == (se > 1)
+            sim.step(nsteps)
+            sim.sync()
+            out.append(([np.asarray(a, np.float32) for a in sim.particles(sync=False)],
+                        np.asarray(sim.collision_counts())))
+    (a, ca), (b, cb) = out
+    assert np.array_equal(ca, cb) and ca[2] > 0
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
