@@ -181,3 +181,37 @@ def test_engine_mcc_births_through_the_fused_sort():
     assert np.array_equal(ca, cb) and ca[2] > 0
     for u, v in zip(a, b):
         assert np.array_equal(u, v)
+
+
+def test_batch_with_an_mcc_run_equals_runs_alone():
+    """A batch holding an MCC run (births through the fused sort's tail items, counted in
+    that run's own rows of the batch's count table) and a collisionless run steps each run
+    exactly as it steps alone."""
+    from pic_amd import Batch
+    mcfg, mparts = mcc_case()
+    lcfg = _lib.default_config(nx=32, ny=32, lx=4.0 * math.pi, ly=4.0 * math.pi,
+                               precision=_lib.PIC_FP32, seed=11)
+    lparts = load_landau(32, 32, lcfg.lx, lcfg.ly, 8, vth=1.0, alpha=0.05, mode=1, seed=11)
+    lcfg.capacity = lparts[0].size
+    specs = [(mcfg, mparts), (lcfg, lparts)]
+    nsteps = 13
+    alone = []
+    for c, p in specs:
+        with Simulation(c) as sim:
+            sim.set_particles(*p)
+            sim.step(nsteps)
+            sim.sync()
+            alone.append(([np.asarray(a, np.float32) for a in sim.particles(sync=False)],
+                          np.asarray(sim.collision_counts())))
+    with Batch([c for c, _ in specs]) as b:
+        for r, (_, p) in enumerate(specs):
+            b.set_particles(*p, run=r)
+        assert b.sort_fused()
+        b.step(nsteps)
+        b.sync()
+        for r in range(len(specs)):
+            got = [np.asarray(a, np.float32) for a in b.particles(run=r, sync=False)]
+            assert np.array_equal(np.asarray(b.collision_counts(run=r)), alone[r][1])
+            for u, v in zip(got, alone[r][0]):
+                assert np.array_equal(u, v)
+    assert alone[0][1][2] > 0  # the MCC run had births
