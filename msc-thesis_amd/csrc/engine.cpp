@@ -515,6 +515,7 @@ int Ctx::launch_collide_stage() {
       a.coll_list = coll_list; a.coll_count = coll_count; a.coll_partials = coll_partials;
       a.count_table = collide_count_table;
       a.from_state_step = 1;
+      a.finish_after = finish_late() ? 1 : 0;
       a.diag = diag + d.dbase * 8;
       launch_mcc(stream, a, d.pcap);
       launch_mcc_births(stream, a, sb.scratch, rho, gsum, d_runs, r, d.pcap);
@@ -567,8 +568,14 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
   }
   launch_push_stage(0, smode);
   if (ev) (void)hipEventRecord(ev[7], stream);  // the push stage is the k_push launch alone
-  if (defer_tail) fin_pending = true;
-  else launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
+  // the step's end (diagnostics row, step counter): before the collision stage with DSMC,
+  // which reads the advanced counter; after it with MCC alone, whose births then enter the
+  // row's particle count and whose end can fold into the next step like a collisionless one
+  auto finish = [&]() {
+    if (defer_tail) fin_pending = true;
+    else launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
+  };
+  if (!finish_late()) finish();
   // with MCC the count table is scanned after the collision stage, whose births add their
   // tiles to it
   const bool scan_after_collide = any_mcc;
@@ -588,6 +595,7 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
   collide_count_table = (smode == 1) ? sb.ftable : nullptr;
   launch_collide_stage();
   collide_count_table = nullptr;
+  if (finish_late()) finish();
   if (smode == 1 && !defer_tail && scan_after_collide)
     launch_fused_scan(stream, d_runs, d_st, nruns, sb, toffs[1 - cur]);
   if (ev) (void)hipEventRecord(ev[5], stream);
@@ -611,9 +619,9 @@ int Ctx::capture_macro(int which) {
   hipGraph_t g = nullptr;
   PIC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   // inside the sequence each step's end-of-step work runs in the next step's field kernels
-  // (the collision stages read the step counter after the finish: not with collisions);
-  // the last step ends with its own kernels, so a replay starts and ends like eager steps
-  const bool fold = !any_mcc && !any_dsmc;
+  // (not with DSMC: its stage reads the counter the step's end advances); the last step
+  // ends with its own kernels, so a replay starts and ends like eager steps
+  const bool fold = !any_dsmc;
   for (int i = 0; i < k; ++i) launch_one_step(true, nullptr, fold && i + 1 < k);
   hipError_t e = hipStreamEndCapture(stream, &g);
   const int flip = cur ^ saved_cur;  // the sorts inside switch buffers (one per sort)
@@ -824,6 +832,13 @@ int set_particles_impl(pic_ctx* ctx, int run, const void* px_, const void* py_,
     PIC_HIP(hipMemcpy(x.gmax + d.cbase, g.data(), sizeof(float) * (G + 1), hipMemcpyHostToDevice));
   }
   PIC_HIP(hipMemsetAsync(x.d_coll_counts + 4 * run, 0, sizeof(long long) * 4, x.stream));
+  {
+    // the first step's collision count starts at zero (each step's end clears the next's)
+    const double zero = 0.0;
+    PIC_HIP(hipMemcpyAsync(x.diag + (d.dbase + s.step % PIC_DIAG_CAP) * 8 + PIC_DIAG_NCOLL,
+                           &zero, sizeof(double), hipMemcpyHostToDevice, x.stream));
+    PIC_HIP(hipStreamSynchronize(x.stream));
+  }
   x.launch_sort_stage();
   x.steps_since_sort = 0;
   x.has_particles[run] = 1;

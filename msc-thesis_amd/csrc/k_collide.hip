@@ -35,7 +35,7 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_null(MccArgs a, int n_upper, int
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   const int np = min(a.st->np, n_upper);
-  const long long step = a.from_state_step ? a.st->step - 1 : a.step;
+  const long long step = a.from_state_step ? a.st->step - (a.finish_after ? 0 : 1) : a.step;
   const uint32_t s32 = (uint32_t)step;
   const uint32_t nk = (uint32_t)a.seed ^ (uint32_t)(a.seed >> 32) ^ PIC_TAG_MCC;
   const int lane = threadIdx.x & 63;
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc(MccArgs a, int per_block) {
   __syncthreads();
   const int ncand = a.coll_count[blockIdx.x];
   const int* list = a.coll_list + (long long)blockIdx.x * per_block;
-  const long long step = a.from_state_step ? a.st->step - 1 : a.step;
+  const long long step = a.from_state_step ? a.st->step - (a.finish_after ? 0 : 1) : a.step;
   const uint32_t s32 = (uint32_t)step;
   for (int k = threadIdx.x; k < ncand; k += PIC_WG) {
     const long long i = a.pbase + list[k];
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
       if (a.counts)
         for (int k = 0; k < 4; ++k) a.counts[k] += (long long)t[k];
       if (a.diag) {
-        const long long step = a.from_state_step ? a.st->step - 1 : a.step;
+        const long long step = a.from_state_step ? a.st->step - (a.finish_after ? 0 : 1) : a.step;
         a.diag[(step % PIC_DIAG_CAP) * 8 + 6] += (double)(t[0] + t[1] + t[2]);
       }
     }
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(PIC_WG) void k_mcc_births(MccArgs a, unsigned long 
   const long long nid = a.st->next_id;
   const int room = max(0, a.cap - np);
   const int take = min(nb, room);
-  const int par = (int)(a.st->step & 1);
+  const int par = (int)((a.st->step + (a.finish_after ? 1 : 0)) & 1);  // the next step's charge
   unsigned long long* rh = rho + par * gsum + rd.gbase;
   const float idx = (float)(1.0 / rd.dx), idy = (float)(1.0 / rd.dy);
   for (int t = threadIdx.x; t < take; t += PIC_WG) {

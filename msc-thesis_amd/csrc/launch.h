@@ -232,6 +232,8 @@ struct MccArgs {
   int* birth_parent;              // [cap]
   float* birth_buf;               // [4*cap]
   int from_state_step;            // 1: take the step index from st->step
+  int finish_after;               // 1: the step's end (counter advance) runs after this
+                                  // stage, so st->step is this step's index (else its + 1)
   long long step;
   double* diag;                   // the run's diagnostics rows (may be NULL): the step's
                                   // collisions are added to column PIC_DIAG_NCOLL

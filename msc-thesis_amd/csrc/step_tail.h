@@ -99,7 +99,9 @@ __device__ __forceinline__ void step_finish_run(const RunDesc* __restrict__ runs
     row[2] = mp * s1;
     row[3] = mp * s2;
     row[5] = (double)np;
-    row[6] = 0.0;  // the collision stage of this step adds its count
+    // the collision count of the next step starts at zero (a collision stage adds to its
+    // step's row, before or after that step's end runs)
+    diag[(rd.dbase + ((stp + 1) % PIC_DIAG_CAP)) * 8 + 6] = 0.0;
     if (advance) {
       st[run].step = stp + 1;
       st[run].vsync = 0;
