@@ -536,6 +536,7 @@ int Ctx::launch_collide_stage() {
       // the cell sort numbers cells tile-major; gmax and the streams are row-major
       a.tile_shift = __builtin_ctz((unsigned)d.tile); a.tiles_y = d.tiles_y; a.ny = d.ny;
       a.st = d_st + r; a.ncoll = d_coll_counts + 4 * r;
+      a.finish_after = finish_late() ? 1 : 0;
       a.diag = diag + d.dbase * 8;
       launch_dsmc(stream, a);
     }
@@ -568,9 +569,9 @@ int Ctx::launch_one_step(bool allow_sort, hipEvent_t* ev, bool defer_tail) {
   }
   launch_push_stage(0, smode);
   if (ev) (void)hipEventRecord(ev[7], stream);  // the push stage is the k_push launch alone
-  // the step's end (diagnostics row, step counter): before the collision stage with DSMC,
-  // which reads the advanced counter; after it with MCC alone, whose births then enter the
-  // row's particle count and whose end can fold into the next step like a collisionless one
+  // the step's end (diagnostics row, step counter) runs after the collision stage: MCC
+  // births then enter the row's particle count, and inside a captured sequence the end
+  // folds into the next step's field kernels as in a collisionless run
   auto finish = [&]() {
     if (defer_tail) fin_pending = true;
     else launch_step_finish(stream, d_runs, d_st, nruns, partials_push, partials_rows, diag);
@@ -618,10 +619,9 @@ int Ctx::capture_macro(int which) {
   const int saved_cur = cur, saved_since = steps_since_sort;
   hipGraph_t g = nullptr;
   PIC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  // inside the sequence each step's end-of-step work runs in the next step's field kernels
-  // (not with DSMC: its stage reads the counter the step's end advances); the last step
-  // ends with its own kernels, so a replay starts and ends like eager steps
-  const bool fold = !any_dsmc;
+  // inside the sequence each step's end-of-step work runs in the next step's field kernels;
+  // the last step ends with its own kernels, so a replay starts and ends like eager steps
+  const bool fold = true;
   for (int i = 0; i < k; ++i) launch_one_step(true, nullptr, fold && i + 1 < k);
   hipError_t e = hipStreamEndCapture(stream, &g);
   const int flip = cur ^ saved_cur;  // the sorts inside switch buffers (one per sort)

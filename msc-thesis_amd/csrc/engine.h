@@ -136,8 +136,8 @@ struct Ctx {
   int capture_macro(int which);
   // steps per captured sequence: a sort cycle, or (DSMC: a cell sort every step) eight steps
   int macro_len() const { return any_dsmc ? 8 : sort_every; }
-  // a step's end runs after its collision stage (MCC without DSMC in the context)
-  bool finish_late() const { return any_mcc && !any_dsmc; }
+  // a step's end runs after its collision stage (any collisions in the context)
+  bool finish_late() const { return any_mcc || any_dsmc; }
 };
 
 }  // namespace pic

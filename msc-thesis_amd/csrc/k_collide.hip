@@ -371,7 +371,7 @@ __global__ __launch_bounds__(DSMC_THREADS) void k_dsmc(DsmcArgs a) {
   __shared__ unsigned long long s_gr[DSMC_CELLS];  // bits of the largest g (g >= 0: ordered)
   __shared__ unsigned long long s_more;
   const int tid = threadIdx.x;
-  const long long step = a.st ? a.st->step - 1 : a.step;
+  const long long step = a.st ? a.st->step - (a.finish_after ? 0 : 1) : a.step;
   const uint32_t s32 = (uint32_t)step;
   const uint32_t sk0 = (uint32_t)a.seed, sk1 = (uint32_t)(a.seed >> 32);
   unsigned int acc = 0;

@@ -265,6 +265,8 @@ struct DsmcArgs {
   uint64_t seed;
   long long step;
   const RunState* st;             // may be NULL (op mode uses `step`)
+  int finish_after;               // 1: the step's end runs after this stage (st->step is this
+                                  // step's index; else its + 1)
   long long* ncoll;
   double* diag;                   // the run's diagnostics rows (may be NULL)
 };
