@@ -170,39 +170,35 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_local(const RunDesc* __restrict
   if (threadIdx.x == PIC_WG - 1) chunk_sums[blockIdx.x] = wpre + inc;
 }
 
-// one workgroup per run: exclusive scan of the chunk sums, total -> offsets[G] (and the
-// closing tile offset when toff is given), nsorted; the first workgroup also clears the
-// count of cells too large for the within-cell sort's small kernel (appended to later)
-__global__ __launch_bounds__(PIC_WG) void k_scan_chunks(const RunDesc* __restrict__ runs,
-                                                        RunState* __restrict__ st,
-                                                        int* __restrict__ chunk_sums,
-                                                        int* __restrict__ offsets,
-                                                        int* __restrict__ toff,
-                                                        int* __restrict__ big_count) {
-  __shared__ int wsum[4];
-  const RunDesc rd = runs[blockIdx.x];
-  const int total = block_exclusive_scan(chunk_sums + rd.cell_item_begin,
-                                         rd.cell_item_end - rd.cell_item_begin, wsum);
-  if (threadIdx.x == 0) {
-    offsets[rd.cbase + rd.nx * rd.ny] = total;
-    st[blockIdx.x].nsorted = total;
-    if (toff) toff[rd.toffbase + rd.nbuckets] = total;
-    if (blockIdx.x == 0) *big_count = 0;
-  }
-}
-
-// the chunk's base added to its offsets; with toff (a tile-major cell numbering: tile t owns
-// cells [t T^2, (t + 1) T^2)) also the tile offsets toff[t] = offsets[t T^2]
+// The chunk's base added to its offsets: the base is the sum of the run's chunk totals
+// before it (integer adds, so any order gives the same base), added up by the workgroup
+// itself, which saves a launch that scanned the chunk totals.  With toff (a tile-major cell
+// numbering: tile t owns cells [t T^2, (t + 1) T^2)) also the tile offsets toff[t] =
+// offsets[t T^2].  The run's last chunk writes the total: offsets[G], nsorted and the
+// closing tile offset; the first workgroup clears the count of cells too large for the
+// within-cell sort's small kernel (which appends to it next).
 __global__ __launch_bounds__(PIC_WG) void k_scan_add(const RunDesc* __restrict__ runs,
+                                                     RunState* __restrict__ st,
                                                      int* __restrict__ offsets,
                                                      const int* __restrict__ chunk_sums,
                                                      const int2* __restrict__ items,
-                                                     int* __restrict__ toff) {
+                                                     int* __restrict__ toff,
+                                                     int* __restrict__ big_count) {
+  __shared__ int wsum[PIC_WG / 64];
   const int2 it = items[blockIdx.x];
   const RunDesc rd = runs[it.x];
   const int G = rd.nx * rd.ny;
   const int TT = rd.tile * rd.tile;
-  const int add = chunk_sums[blockIdx.x];
+  int part = 0;
+  for (int i = rd.cell_item_begin + (int)threadIdx.x; i < (int)blockIdx.x; i += PIC_WG)
+    part += chunk_sums[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
+  __syncthreads();
+  int add = 0;
+#pragma unroll
+  for (int w = 0; w < PIC_WG / 64; ++w) add += wsum[w];
   for (int k = threadIdx.x; k < PIC_CELL_CHUNK; k += PIC_WG) {
     int c = it.y + k;
     if (c < G) {
@@ -210,6 +206,15 @@ __global__ __launch_bounds__(PIC_WG) void k_scan_add(const RunDesc* __restrict__
       offsets[rd.cbase + c] = o;
       if (toff && c % TT == 0) toff[rd.toffbase + c / TT] = o;
     }
+  }
+  if (threadIdx.x == 0) {
+    if ((int)blockIdx.x == rd.cell_item_end - 1) {
+      const int total = add + chunk_sums[blockIdx.x];
+      offsets[rd.cbase + G] = total;
+      st[it.x].nsorted = total;
+      if (toff) toff[rd.toffbase + rd.nbuckets] = total;
+    }
+    if (blockIdx.x == 0) *big_count = 0;
   }
 }
 
@@ -1011,9 +1016,8 @@ static void sort_scan(hipStream_t s, const RunDesc* runs, RunState* st, int nrun
                       const int2* citems, int ncitems, int clear, int* toff) {
   k_scan_local<<<ncitems, PIC_WG, 0, s>>>(runs, sb.counts, sb.offsets, sb.chunk_sums, citems,
                                           clear);
-  k_scan_chunks<<<nruns, PIC_WG, 0, s>>>(runs, st, sb.chunk_sums, sb.offsets, toff,
-                                         sb.big_count);
-  k_scan_add<<<ncitems, PIC_WG, 0, s>>>(runs, sb.offsets, sb.chunk_sums, citems, toff);
+  k_scan_add<<<ncitems, PIC_WG, 0, s>>>(runs, st, sb.offsets, sb.chunk_sums, citems, toff,
+                                        sb.big_count);
 }
 
 template <typename R>
