@@ -185,7 +185,7 @@ def _oracle():
 
 def two_stream_rate(warmup=10):
     """Config 1: cold two-stream, 64 x 64, 409,600 FP64 particles, FP64 everything, dt = 0.1,
-    the configured 200 steps timed after warm-up steps; the growth rate of |rho_hat(k)| fitted
+    autotuned, the configured 200 steps timed after warm-up steps; the growth rate of |rho_hat(k)| fitted
     over the linear phase t in [8, 16] of the timed run (cold beams, k v0 = 0.5:
     gamma = 0.3406).  CPU baseline: the oracle's FP64 step (or_run_f64, one thread) over the
     whole 200-step run on the same particles."""
@@ -196,6 +196,8 @@ def two_stream_rate(warmup=10):
     n = parts[0].size
     with Simulation(cfg) as sim:
         sim.set_particles(*parts)
+        sim.autotune(8)  # tiling / splits / sort cadence (the state is restored afterwards)
+        tune = sim.tuning()
         sim.step(warmup)
         sim.sync()
         torch.cuda.synchronize()
@@ -212,7 +214,7 @@ def two_stream_rate(warmup=10):
     gamma = float(np.polyfit(t[m], np.log(diag[m, 4]), 1)[0])
     out = {"particle_steps_per_s": n * nsteps / dt, "ms_per_step": dt * 1e3 / nsteps,
            "particles": n, "steps": nsteps, "grid": [cfg.nx, cfg.ny], "dtype": "f64",
-           "growth_rate": gamma, "growth_rate_theory": 0.3406}
+           "growth_rate": gamma, "growth_rate_theory": 0.3406, "tuning": tune}
     return out
 
 

@@ -1183,8 +1183,9 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
   double best = 1e30;
   int bt = 0, bh = 0, bs = 0;
   const int tiles[3] = {8, 16, 32}, halos[2] = {8, 16}, cad[5] = {1, 2, 4, 6, 8};
-  // splits: 0 = sized by particles per tile (the only choice for ragged batches)
-  const int splitc[3] = {0, 1, 2};
+  // splits: 0 = sized by particles per tile (the only choice for ragged batches); the
+  // larger counts fill the GPU on small grids (a 64^2 run has 16 tiles of 16^2 cells)
+  const int splitc[6] = {0, 1, 2, 4, 8, 16};
   int rc = PIC_OK;
   int bsp = 0;
   for (int ti : tiles) {
@@ -1194,6 +1195,10 @@ int pic_autotune(pic_ctx* ctx, int trial_steps, double* best_ms) {
       for (int sp : splitc) {
         if (x.any_dsmc && se != 1) continue;
         if (sp && x.nruns > 1) continue;
+        // at least ~1k particles per workgroup for the larger split counts
+        if (sp > 2 && (long long)x.cfg[0].capacity <
+                          1024LL * sp * (x.cfg[0].nx / ti) * (x.cfg[0].ny / ti))
+          continue;
         rc = restore();
         if (rc) break;
         for (int r = 0; r < x.nruns; ++r) x.tune[r] = default_tuning(x.cfg[r], ti, h, sp);
