@@ -154,13 +154,16 @@ def stream_probe(reps=20, n=1 << 24):
 
 def mcc_rate(steps=100, warmup=10):
     """Config 3: electron-neutral null-collision MCC on 256 x 256 with 8,388,608 electrons
-    (Te = 2 eV, neutrals 1e21 m^-3), the preset's tiling and sort cadence; particle-steps/s
-    of the whole step (field, push, collisions, births, sort) after warm-up steps."""
+    (Te = 2 eV, neutrals 1e21 m^-3), autotuned (tile, halo, splits, sort cadence; the
+    state is restored afterwards); particle-steps/s of the whole step (field, push,
+    collisions, births, sort) after warm-up steps."""
     import torch
     from pic_amd import Simulation, presets
     cfg, parts, _ = presets.mcc_256()
     with Simulation(cfg) as sim:
         sim.set_particles(*parts)
+        sim.autotune(8)
+        tune = sim.tuning()
         sim.step(warmup)
         sim.sync()
         torch.cuda.synchronize()
@@ -174,7 +177,7 @@ def mcc_rate(steps=100, warmup=10):
         n1 = sim.num_particles()
     return {"particle_steps_per_s": 0.5 * (n0 + n1) * steps / dt, "ms_per_step": dt * 1e3 / steps,
             "particles": n1, "steps": steps, "collision_counts": counts,
-            "grid": [cfg.nx, cfg.ny]}
+            "grid": [cfg.nx, cfg.ny], "tuning": tune}
 
 
 def _oracle():
